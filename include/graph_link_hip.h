@@ -1,0 +1,189 @@
+/*
+ * graph_link_hip.h — C ABI of the MI355X (gfx950) Personalized-PageRank hot path.
+ *
+ * Every entry point:
+ *   - takes plain device pointers, int64 sizes and an explicit hipStream_t (passed as void*);
+ *   - returns GL_OK (0) or a GL_ERR_* code, with a message available from gl_last_error();
+ *   - never throws across the ABI and never allocates device memory: scratch space is a
+ *     caller-provided workspace whose size comes from the matching *_workspace_bytes query.
+ *
+ * Matrix layout on the device is the reference's PBR ("packed block representation"):
+ *   codes    u64[T]     occupancy bitmask of each BRxBC tile, bit (r*BC + c)
+ *   coords   i32[2T]    (row0, col0) of each tile, multiples of BR / BC
+ *   offsets  i32[T+1]   prefix of popcounts: tile t owns data[offsets[t] .. offsets[t+1])
+ *   data     f32[V]     packed tile values in bit order
+ *   rem_*    CSR        indptr i32[rows+1], col i32[R], val f32[R] for the tiles below
+ *                       min_nnz_per_block
+ * plus one derived index, stripe_ptr i32[num_stripes+1] (tiles of block row s are
+ * [stripe_ptr[s], stripe_ptr[s+1]) — tiles are stored stripe-ascending by every builder).
+ * Dense right-hand sides X and results Y are row-major [rows][K] with leading dimension ld.
+ */
+#ifndef GRAPH_LINK_HIP_H
+#define GRAPH_LINK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GL_OK 0
+#define GL_ERR_INVALID 1     /* bad argument (shape, pointer, unsupported block size)   */
+#define GL_ERR_HIP 2         /* a HIP runtime call or kernel launch failed              */
+#define GL_ERR_WORKSPACE 3   /* caller workspace too small                              */
+#define GL_ERR_NOTCONVERGED 4 /* not an error code of any call; reserved for callers    */
+
+/* Last error message of the calling thread ("" when none). */
+const char* gl_last_error(void);
+/* ABI version; bumped on any signature change. */
+int32_t gl_abi_version(void);
+/* Compute units and LDS bytes of the current device (for launch sizing and reports). */
+int gl_device_info(int32_t* num_cus, int32_t* lds_bytes_per_cu, int32_t* clock_khz, int64_t* hbm_bytes);
+
+/* ------------------------------------------------------------------------------------
+ * Derived index of a PBR matrix.
+ * stripe_ptr[s] = first tile whose coords row0 / block_rows >= s, for s in [0, num_stripes].
+ * Replaces nothing in the reference (its kernels launch one block per tile,
+ * Backend/graph_link/kernels/pbr_kernel.cu:357-364); computed once per upload.
+ * ---------------------------------------------------------------------------------- */
+int gl_pbr_stripe_index(int64_t rows, int32_t block_rows, int64_t num_tiles,
+                        const int32_t* coords, int32_t* stripe_ptr, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * SpMM  Y = A X  (or Y += A X when accumulate != 0), A in PBR, X/Y row-major [rows][K].
+ * Replaces pbr_spmm_cuda_{i32}_{f32} (Backend/graph_link/bindings/bindings_kernels.cu:13-83,
+ * bound at :142-145) and the launchers launch_pbr_spmm / launch_csr_spmm
+ * (Backend/graph_link/kernels/pbr_kernel.cu:336-396, :460-478).
+ *   mode GL_SPMM_FUSED : one stripe-owner kernel does tiles + remainder, plain stores.
+ *   mode GL_SPMM_SPLIT : block kernel (b) on `stream`, CSR-remainder kernel (c) on the
+ *                        library's side stream, joined back to `stream` with events.
+ * kw: RHS columns per launch (1,2,4,8 bit-lane kernel; 16,32,64 feature-lane kernel);
+ *     0 lets the library choose. Block sizes 2x2, 4x4, 8x8 (as the reference).
+ * ---------------------------------------------------------------------------------- */
+#define GL_SPMM_FUSED 0
+#define GL_SPMM_SPLIT 1
+int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                        int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                        const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
+                        const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                        const float* rem_vals, const float* X, int64_t ldx, int64_t K,
+                        float* Y, int64_t ldy, int32_t accumulate, int32_t mode, int32_t kw,
+                        void* stream);
+
+/* Block-tile part alone (north-star subsystem b). Rows with no tile get 0 (or keep Y). */
+int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
+                              int32_t block_cols, int64_t num_tiles, const uint64_t* codes,
+                              const int32_t* coords, const int32_t* offsets, const float* data,
+                              const int32_t* stripe_ptr, const float* X, int64_t ldx,
+                              int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
+                              void* stream);
+
+/* CSR SpMM alone (north-star subsystem c; the PBR remainder, or any CSR matrix).
+ * Replaces launch_csr_spmm (Backend/graph_link/kernels/pbr_kernel.cu:404-478). */
+int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, const int32_t* indptr,
+                        const int32_t* col_ind, const float* vals, const float* X, int64_t ldx,
+                        int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
+                        void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Personalized PageRank power iteration (north-star subsystem d + the persistent loop).
+ * Replaces run_personalized_pagerank (Backend/graph_link/graph_link/__init__.py:210-271),
+ * init_ppr (Backend/graph_link/kernels/ppr_kernel.cu:199-216), ppr_update_normalized
+ * (:130-187) and the dangling->seed step missing_mass + ppr_update (:13-117).
+ *
+ * Column k of X is the PPR vector of seed set k: personalization p_k[i] = seed_w[e] for the
+ * entries e in [seed_ptr[k], seed_ptr[k+1]) with seed_idx[e] == i (weights of one set sum
+ * to 1; duplicate nodes are allowed and add up). Seed arrays are HOST arrays.
+ *   X_{t+1} = alpha * A X_t + alpha * dangling_k + (1 - alpha) p_k
+ * dangling_mode: GL_DANGLING_NONE (A is column-stochastic already, as the reference's
+ *   normalize_transition_matrix makes it), GL_DANGLING_UNIFORM (mass of rows flagged in
+ *   dangling_mask spread as 1/N), GL_DANGLING_PERSONALIZATION (that mass returns to p_k).
+ * retire_mode: GL_RETIRE_TOGETHER stops every column when max_k residual_k < tol (the
+ *   reference); GL_RETIRE_PER_COLUMN freezes each column at its own residual_k < tol.
+ * residual_k = sum_i |X_{t+1}[i,k] - X_t[i,k]| (L1), as the reference.
+ * X0: warm start [rows][K] (NULL = seed one-hot start, as the reference).
+ * Outputs: X_out [rows][K] (ld = K); iterations[K], converged[K] (device int32);
+ *   history [max_iter][K] residuals (device, may be NULL); *total_iterations (host).
+ * graph_iters > 0 captures that many iterations into one hipGraph and polls the device
+ *   stop flag once per replay; 0 launches eagerly and polls every iteration.
+ * ---------------------------------------------------------------------------------- */
+#define GL_DANGLING_NONE 0
+#define GL_DANGLING_UNIFORM 1
+#define GL_DANGLING_PERSONALIZATION 2
+#define GL_RETIRE_TOGETHER 0
+#define GL_RETIRE_PER_COLUMN 1
+
+typedef struct gl_ppr_opts {
+  float alpha;
+  float tol;
+  int32_t max_iter;
+  int32_t dangling_mode;
+  int32_t retire_mode;
+  int32_t graph_iters;
+  int32_t kw;          /* RHS columns per launch, 0 = library choice */
+  int32_t grid_mult;   /* workgroups per CU of the persistent launch, 0 = library choice */
+} gl_ppr_opts;
+
+int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,
+                               int64_t num_seed_entries);
+
+int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                       int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                       const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
+                       const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                       const float* rem_vals, int64_t K, const int32_t* seed_ptr,
+                       const int32_t* seed_idx, const float* seed_w,
+                       const uint8_t* dangling_mask, const float* X0, const gl_ppr_opts* opts,
+                       float* X_out, int32_t* iterations, int32_t* converged, float* history,
+                       void* workspace, int64_t workspace_bytes, int32_t* total_iterations,
+                       void* stream);
+
+
+/* ------------------------------------------------------------------------------------
+ * GPU PBR builder (north-star subsystem a). Replaces csr_to_pbr
+ * (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477, wrapper
+ * Backend/graph_link/graph_link/__init__.py:137-156). Input: canonical CSR on the device
+ * (int32 indptr/indices, rows sorted by column, no duplicates); values are copied as opaque
+ * 4- or 8-byte payloads, so float32 and float64 matrices build bit-exactly.
+ * Two calls on the same workspace:
+ *   gl_pbr_build_count: fills rem_indptr[rows+1] and returns counts[4] = {T, V, R, D}
+ *                       (tiles, tile values, remainder entries, distinct blocks) on the host;
+ *   gl_pbr_build_fill : writes codes[T], coords[2T], offsets[T+1], data[V], rem_col_ind[R],
+ *                       rem_vals[R] in canonical order (block columns ascending inside a
+ *                       stripe) and, when blk_bc != NULL, the distinct-block list
+ *                       blk_ptr[ns+1], blk_bc[D], blk_rmin[D], blk_tile[D] used to restore
+ *                       the reference's emission order.
+ * Block sizes: any block_rows x block_cols <= 64 (the reference's bitset<64> code).
+ * ---------------------------------------------------------------------------------- */
+int64_t gl_pbr_build_workspace_bytes(int64_t rows, int32_t block_rows);
+int gl_pbr_build_count(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                       int32_t min_nnz_per_block, const int32_t* indptr,
+                       const int32_t* indices, int32_t* rem_indptr, void* workspace,
+                       int64_t workspace_bytes, int64_t* counts, void* stream);
+int gl_pbr_build_fill(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                      int32_t min_nnz_per_block, int64_t num_tiles, const int32_t* indptr,
+                      const int32_t* indices, const void* values, int32_t value_bytes,
+                      void* workspace, int64_t workspace_bytes, uint64_t* codes,
+                      int32_t* coords, int32_t* offsets, void* data, const int32_t* rem_indptr,
+                      int32_t* rem_col_ind, void* rem_vals, int32_t* blk_ptr, int32_t* blk_bc,
+                      int8_t* blk_rmin, int32_t* blk_tile, void* stream);
+/* Reference emission order: replays the reference builder's std::unordered_map<index_t,...>
+ * (bucket hint sqrt(num_col_blocks), inserts in CSR first-occurrence order, clear() per
+ * stripe; pbr_matrix.cpp:374-428) on HOST copies of the block list and returns perm[T]
+ * (new tile -> canonical tile). index_bytes = 4 or 8 selects the key type. */
+int gl_pbr_reference_perm(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                          int32_t index_bytes, const int32_t* blk_ptr, const int32_t* blk_bc,
+                          const int8_t* blk_rmin, const int32_t* blk_tile, int32_t* perm);
+/* Device gather of tiles into perm order (values move with their tile). */
+int64_t gl_pbr_permute_workspace_bytes(int64_t num_tiles);
+int gl_pbr_permute_tiles(int64_t num_tiles, const int32_t* perm, const uint64_t* codes,
+                         const int32_t* coords, const int32_t* offsets, const void* data,
+                         int32_t value_bytes, uint64_t* codes_out, int32_t* coords_out,
+                         int32_t* offsets_out, void* data_out, void* workspace,
+                         int64_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPH_LINK_HIP_H */

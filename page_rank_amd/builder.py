@@ -1,0 +1,134 @@
+"""csr_to_pbr on the GPU (Backend/graph_link/graph_link/__init__.py:137-156 ->
+Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477), and pbr_to_csr.
+
+The matrix is built by the HIP builder of libgraphlink_hip.so on the current device and stays
+resident there (the returned object is already "on cuda"); host arrays are fetched lazily.
+order="reference" (default) reproduces the reference's tile order inside each stripe
+bit-for-bit; order="canonical" keeps block columns ascending (no host replay).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _native
+from .pbr import pbr_class, stripe_index
+
+
+def _as_csr_arrays(mat):
+    if isinstance(mat, torch.Tensor):
+        if mat.device.type != "cpu":
+            raise ValueError("Input tensor must be on CPU")
+        mat = sp.csr_matrix((mat.values().numpy(), mat.col_indices().numpy(),
+                             mat.crow_indices().numpy()), shape=tuple(mat.shape))
+    elif not isinstance(mat, sp.csr_matrix):
+        raise TypeError(f"Unsupported type {type(mat)}")
+    return mat
+
+
+def _index_dtype(mat):
+    it = np.dtype(mat.indices.dtype)
+    return it if it in (np.dtype(np.int32), np.dtype(np.int64)) else np.dtype(np.int64)
+
+
+def _scalar_dtype(mat):
+    st = np.dtype(mat.data.dtype)
+    return st if st in (np.dtype(np.float32), np.dtype(np.float64)) else np.dtype(np.float32)
+
+
+def csr_to_pbr(mat, block_rows: int = 2, block_cols: int = 2, min_nnz_per_block: int = 2, *,
+               order: str = "reference", device=None):
+    """Build a PBR matrix from a canonical CSR matrix on the GPU."""
+    mat = _as_csr_arrays(mat)
+    if order not in ("reference", "canonical"):
+        raise ValueError("order must be 'reference' or 'canonical'")
+    it, st = _index_dtype(mat), _scalar_dtype(mat)
+    cls = pbr_class(it, st)
+    rows, cols = mat.shape
+    if block_rows < 1 or block_cols < 1 or block_rows * block_cols > 64:
+        raise ValueError(f"block {block_rows}x{block_cols} does not fit a 64-bit code")
+    if not 0 <= min_nnz_per_block <= 255:
+        raise ValueError("min_nnz_per_block must be in [0, 255]")
+    if max(rows, cols, mat.nnz) >= 2**31 - 1:
+        raise ValueError("matrix too large for int32 device indices")
+    if not mat.has_canonical_format:
+        # the builder needs sorted, duplicate-free rows (the reference silently produces
+        # misaligned values otherwise)
+        mat = mat.copy()
+        mat.sum_duplicates()
+    L = _native.lib()
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    vb = 8 if st == np.float64 else 4
+    with torch.cuda.device(dev):
+        stream = _native.stream_handle(dev)
+        indptr = torch.as_tensor(mat.indptr.astype(np.int32, copy=False)).to(dev)
+        indices = torch.as_tensor(mat.indices.astype(np.int32, copy=False)).to(dev)
+        vals = torch.as_tensor(np.ascontiguousarray(mat.data.astype(st, copy=False))).to(dev)
+        wsb = L.gl_pbr_build_workspace_bytes(rows, block_rows)
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+        rem_indptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
+        counts = np.zeros(4, dtype=np.int64)
+        _native.check(L.gl_pbr_build_count(rows, cols, block_rows, block_cols,
+                                           int(min_nnz_per_block), indptr.data_ptr(),
+                                           indices.data_ptr(), rem_indptr.data_ptr(),
+                                           ws.data_ptr(), wsb, counts.ctypes.data, stream),
+                      "gl_pbr_build_count")
+        T, V, R, D = (int(c) for c in counts)
+        ns = (rows + block_rows - 1) // block_rows
+        vdt = torch.float64 if vb == 8 else torch.float32
+        codes = torch.empty(T, dtype=torch.int64, device=dev)
+        coords = torch.empty(2 * T, dtype=torch.int32, device=dev)
+        offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
+        data = torch.empty(V, dtype=vdt, device=dev)
+        rem_col = torch.empty(R, dtype=torch.int32, device=dev)
+        rem_vals = torch.empty(R, dtype=vdt, device=dev)
+        want_blocks = order == "reference" and T > 1
+        if want_blocks:
+            blk_ptr = torch.empty(ns + 1, dtype=torch.int32, device=dev)
+            blk_bc = torch.empty(max(D, 1), dtype=torch.int32, device=dev)
+            blk_rmin = torch.empty(max(D, 1), dtype=torch.int8, device=dev)
+            blk_tile = torch.empty(max(D, 1), dtype=torch.int32, device=dev)
+        else:
+            blk_ptr = blk_bc = blk_rmin = blk_tile = None
+        p = _native.ptr
+        _native.check(L.gl_pbr_build_fill(
+            rows, cols, block_rows, block_cols, int(min_nnz_per_block), T, indptr.data_ptr(),
+            indices.data_ptr(), vals.data_ptr(), vb, ws.data_ptr(), wsb, p(codes), p(coords),
+            offsets.data_ptr(), p(data), rem_indptr.data_ptr(), p(rem_col), p(rem_vals),
+            p(blk_ptr), p(blk_bc), p(blk_rmin), p(blk_tile), stream), "gl_pbr_build_fill")
+        if want_blocks:
+            h_ptr = blk_ptr.cpu().numpy()
+            h_bc = blk_bc.cpu().numpy()
+            h_rmin = blk_rmin.cpu().numpy()
+            h_tile = blk_tile.cpu().numpy()
+            perm = np.empty(T, dtype=np.int32)
+            _native.check(L.gl_pbr_reference_perm(rows, cols, block_rows, block_cols,
+                                                  it.itemsize, h_ptr.ctypes.data,
+                                                  h_bc.ctypes.data, h_rmin.ctypes.data,
+                                                  h_tile.ctypes.data, perm.ctypes.data),
+                          "gl_pbr_reference_perm")
+            if not np.array_equal(np.sort(perm), np.arange(T, dtype=np.int32)):
+                raise RuntimeError("reference-order replay did not produce a permutation")
+            perm_d = torch.as_tensor(perm).to(dev)
+            codes2 = torch.empty_like(codes)
+            coords2 = torch.empty_like(coords)
+            offsets2 = torch.empty_like(offsets)
+            data2 = torch.empty_like(data)
+            pwb = L.gl_pbr_permute_workspace_bytes(T)
+            pws = torch.empty(max(pwb, 1), dtype=torch.uint8, device=dev)
+            _native.check(L.gl_pbr_permute_tiles(T, perm_d.data_ptr(), codes.data_ptr(),
+                                                 coords.data_ptr(), offsets.data_ptr(),
+                                                 p(data), vb, codes2.data_ptr(),
+                                                 coords2.data_ptr(), offsets2.data_ptr(),
+                                                 p(data2), pws.data_ptr(), pwb, stream),
+                          "gl_pbr_permute_tiles")
+            codes, coords, offsets, data = codes2, coords2, offsets2, data2
+        meta = {"codes": codes, "coords": coords, "offsets": offsets, "data": data,
+                "rem_indptr": rem_indptr, "rem_col_ind": rem_col, "rem_vals": rem_vals}
+        meta["stripe_ptr"] = stripe_index(coords, rows, block_rows, T, dev)
+    return cls(rows, cols, block_rows, block_cols, mat.nnz, device_meta=meta)
+
+
+def pbr_to_csr(pbr_mat, indptr, indices, data):
+    raise NotImplementedError("pbr_to_csr is not available yet")

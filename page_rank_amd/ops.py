@@ -1,0 +1,75 @@
+"""SpMM entry points: pbr_batched_matmul_cuda / pbr_matmul
+(Backend/graph_link/graph_link/__init__.py:66-134)."""
+from __future__ import annotations
+
+import torch
+
+from . import _native
+from .registry import get_pbr_gpu_meta
+
+
+def _meta_or_raise(pbr_mat):
+    meta = get_pbr_gpu_meta(pbr_mat)
+    if meta is None:
+        raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    return meta
+
+
+def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mode="fused",
+              kw=0):
+    """y (+)= A @ x for one [rows, K] right-hand side, both on the matrix's device."""
+    meta = _meta_or_raise(pbr_mat)
+    if x.dtype != torch.float32 or y.dtype != torch.float32:
+        raise TypeError(f"Unsupported dtype combination: index={meta['coords'].dtype}, "
+                        f"data={x.dtype}")
+    if x.dim() != 2 or x.shape[0] != pbr_mat.cols or y.shape != (pbr_mat.rows, x.shape[1]):
+        raise ValueError(f"shape mismatch: A is {pbr_mat.rows}x{pbr_mat.cols}, x {tuple(x.shape)}"
+                         f", y {tuple(y.shape)}")
+    if x.stride(1) != 1 or y.stride(1) != 1:
+        raise ValueError("x and y must have unit stride along K")
+    K = x.shape[1]
+    L = _native.lib()
+    m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
+    with torch.cuda.device(x.device):
+        rc = L.gl_pbr_spmm_i32_f32(
+            pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
+            meta["codes"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+            meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
+            meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
+            meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), K, y.data_ptr(),
+            y.stride(0), 1 if accumulate else 0, m, kw, _native.stream_handle(x.device))
+    _native.check(rc, "gl_pbr_spmm_i32_f32")
+    return y
+
+
+def pbr_batched_matmul_cuda(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0) -> torch.Tensor:
+    """Y = A @ x with shape [batch, rows, features] (x is [rows, K] or [batch, rows, K])."""
+    meta = _meta_or_raise(pbr_mat)
+    if meta["data"].dtype != torch.float32:
+        raise TypeError(f"Unsupported dtype combination: index={meta['coords'].dtype}, "
+                        f"data={meta['data'].dtype}")
+    x3 = x.unsqueeze(0) if x.dim() == 2 else x
+    x3 = x3.contiguous()
+    y = torch.empty((x3.shape[0], pbr_mat.rows, x3.shape[2]), dtype=x3.dtype, device=x3.device)
+    for b in range(x3.shape[0]):
+        spmm_into(pbr_mat, x3[b], y[b], mode=mode, kw=kw)
+    return y
+
+
+def pbr_matmul(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0):
+    """Unified entry point: x on the GPU -> HIP SpMM (uploading the matrix on first use)."""
+    if x.dim() not in (2, 3):
+        raise ValueError(f"Expected 2D or 3D input, got {x.dim()}D")
+    if not x.is_cuda:
+        raise NotImplementedError(
+            "pbr_matmul: this build runs on the GPU only; move x to a HIP device "
+            "(the reference's CPU SpMM is not part of it)")
+    if get_pbr_gpu_meta(pbr_mat) is None:
+        pbr_mat = pbr_mat.to(x.device)
+    y = pbr_batched_matmul_cuda(pbr_mat, x, mode=mode, kw=kw)
+    return y.squeeze(0) if x.dim() == 2 else y
+
+
+def pbr_batched_matmul_cpu(pbr_mat, x):
+    raise NotImplementedError("pbr_batched_matmul_cpu: this build has no CPU SpMM; use "
+                              "pbr_matmul with a GPU tensor")

@@ -1,0 +1,149 @@
+"""Personalized PageRank drivers.
+
+run_personalized_pagerank  drop-in for Backend/graph_link/graph_link/__init__.py:210-271
+run_ppr                    batched seed sets, dangling handling, per-column retirement,
+                           warm start, hipGraph-captured loop (one C-ABI call)
+normalize_transition_matrix  Backend/graph_link/graph_link/__init__.py:184-207
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _native
+from .registry import get_pbr_gpu_meta
+
+_DANGLING = {"none": _native.GL_DANGLING_NONE, "uniform": _native.GL_DANGLING_UNIFORM,
+             "personalization": _native.GL_DANGLING_PERSONALIZATION}
+_RETIRE = {"together": _native.GL_RETIRE_TOGETHER, "per_column": _native.GL_RETIRE_PER_COLUMN}
+
+
+def normalize_transition_matrix(A_csr: sp.csr_matrix) -> sp.csr_matrix:
+    """Column-stochastic transition matrix: each column divided by its sum (float32);
+    dangling columns get a dense uniform 1/N column, as the reference does."""
+    N = A_csr.shape[0]
+    A = A_csr.tocsc().astype(np.float32)
+    col_sums = np.asarray(A.sum(axis=0), dtype=np.float32).ravel()
+    inv = np.zeros_like(col_sums)
+    np.divide(1.0, col_sums, out=inv, where=col_sums > 0)
+    A_norm = A.multiply(inv).tocsr()
+    dangling = np.where(col_sums == 0)[0]
+    if dangling.size > 0:
+        r = np.tile(np.arange(N, dtype=np.int32), dangling.size)
+        c = np.repeat(dangling.astype(np.int32), N)
+        v = np.full(N * dangling.size, 1.0 / N, dtype=np.float32)
+        A_norm = A_norm + sp.csr_matrix((v, (r, c)), shape=(N, N))
+    return A_norm
+
+
+@dataclass
+class PPRResult:
+    X: torch.Tensor              # [N, K] float32 on the device
+    iterations: np.ndarray       # [K] int32: iteration at which each column stopped
+    converged: np.ndarray        # [K] bool
+    history: np.ndarray          # [total_iterations, K] L1 residual (0 once a column stopped)
+    total_iterations: int
+
+
+def _seed_arrays(seed_sets, weights=None):
+    ptr = [0]
+    idx = []
+    w = []
+    for k, s in enumerate(seed_sets):
+        s = np.atleast_1d(np.asarray(s, dtype=np.int64))
+        if s.size == 0:
+            raise ValueError(f"seed set {k} is empty")
+        idx.append(s)
+        if weights is None:
+            w.append(np.full(s.size, 1.0 / s.size, dtype=np.float32))
+        else:
+            ws = np.asarray(weights[k], dtype=np.float32)
+            if ws.shape != s.shape:
+                raise ValueError("weights must match seed sets")
+            w.append(ws)
+        ptr.append(ptr[-1] + s.size)
+    return (np.asarray(ptr, dtype=np.int32), np.concatenate(idx).astype(np.int32),
+            np.concatenate(w).astype(np.float32))
+
+
+def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=None,
+            dangling_mode="none", retire="per_column", x0=None, weights=None,
+            graph_iters=8, kw=0, grid_mult=0, return_history=True) -> PPRResult:
+    """Batched PPR, one column per seed set (a list of node-index lists)."""
+    meta = get_pbr_gpu_meta(pbr_mat)
+    if meta is None:
+        raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    if meta["data"].dtype != torch.float32:
+        raise TypeError("run_ppr needs a float32 matrix")
+    if pbr_mat.rows != pbr_mat.cols:
+        raise ValueError("PPR needs a square matrix")
+    device = meta["codes"].device
+    N = pbr_mat.rows
+    ptr, idx, w = _seed_arrays(seed_sets, weights)
+    if idx.size and (idx.min() < 0 or idx.max() >= N):
+        raise ValueError("seed node out of range")
+    K = len(seed_sets)
+    dmode = _DANGLING[dangling_mode]
+    dmask = None
+    if dmode != _native.GL_DANGLING_NONE:
+        if dangling is None:
+            raise ValueError(f"dangling_mode={dangling_mode!r} needs the dangling mask")
+        dmask = torch.as_tensor(np.asarray(dangling, dtype=np.uint8)).to(device)
+    if x0 is not None:
+        x0 = torch.as_tensor(x0, dtype=torch.float32, device=device).contiguous()
+        if x0.shape != (N, K):
+            raise ValueError(f"x0 must be [{N}, {K}]")
+    X = torch.empty((N, K), dtype=torch.float32, device=device)
+    iters = torch.zeros(K, dtype=torch.int32, device=device)
+    conv = torch.zeros(K, dtype=torch.int32, device=device)
+    hist = (torch.zeros((max_iter, K), dtype=torch.float32, device=device)
+            if return_history else None)
+    L = _native.lib()
+    wsb = L.gl_ppr_workspace_bytes(N, pbr_mat.block_rows, K, int(idx.size))
+    if wsb < 0:
+        raise ValueError("invalid PPR sizes")
+    ws = torch.empty(wsb, dtype=torch.uint8, device=device)
+    opts = _native.GlOpts(alpha=alpha, tol=tol, max_iter=int(max_iter), dangling_mode=dmode,
+                          retire_mode=_RETIRE[retire], graph_iters=int(graph_iters), kw=int(kw),
+                          grid_mult=int(grid_mult))
+    total = np.zeros(1, dtype=np.int32)
+    with torch.cuda.device(device):
+        rc = L.gl_ppr_run_i32_f32(
+            N, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols, meta["codes"].numel(),
+            meta["codes"].data_ptr(), meta["coords"].data_ptr(), meta["offsets"].data_ptr(),
+            meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
+            meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
+            meta["rem_vals"].data_ptr(), K, ptr.ctypes.data, idx.ctypes.data, w.ctypes.data,
+            _native.ptr(dmask), _native.ptr(x0), opts, X.data_ptr(), iters.data_ptr(),
+            conv.data_ptr(), _native.ptr(hist), ws.data_ptr(), wsb, total.ctypes.data,
+            _native.stream_handle(device))
+    _native.check(rc, "gl_ppr_run_i32_f32")
+    T = int(total[0])
+    return PPRResult(X=X, iterations=iters.cpu().numpy(), converged=conv.cpu().numpy() != 0,
+                     history=(hist[:T].cpu().numpy() if hist is not None
+                              else np.zeros((0, K), np.float32)),
+                     total_iterations=T)
+
+
+def run_personalized_pagerank(pbr_mat, source_nodes: torch.Tensor, alpha: float = 0.85,
+                              max_iterations: int = 100, tolerance: float = 1e-6,
+                              return_history: bool = False):
+    """X_{t+1} = alpha * A @ X_t + (1 - alpha) * e_s for one seed per column, all columns
+    stopping together when the max L1 update is below tolerance. Returns
+    (X, iterations_run, converged[, history]) like the reference."""
+    meta = get_pbr_gpu_meta(pbr_mat)
+    if meta is None:
+        raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    src = torch.as_tensor(source_nodes).to(torch.int32).cpu().numpy().astype(np.int64)
+    res = run_ppr(pbr_mat, [[int(s)] for s in src], alpha=alpha, tol=tolerance,
+                  max_iter=max_iterations, dangling_mode="none", retire="together",
+                  return_history=True)
+    iterations = int(res.iterations.max()) if res.iterations.size else 0
+    converged = bool(res.converged.all())
+    if return_history:
+        history = [float(h) for h in res.history.max(axis=1)]
+        return res.X, iterations, converged, history
+    return res.X, iterations, converged

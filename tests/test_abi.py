@@ -1,0 +1,70 @@
+"""CPU checks of the C ABI: the library loads and exports every function that
+include/graph_link_hip.h declares, and the ctypes table covers exactly that set."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from page_rank_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "graph_link_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(gl_[A-Za-z0-9_]+)\s*\(",
+                       text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "gl_pbr_spmm_i32_f32" in names and "gl_ppr_run_i32_f32" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.fail(f"{_native.LIB_PATH} missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(_native.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    assert sorted(_native.exported_symbols()) == declared_functions()
+
+
+def test_library_loads_and_reports_version():
+    L = _native.lib()
+    assert L.gl_abi_version() == 1
+    assert L.gl_last_error() is not None
+
+
+def test_argument_validation_without_device():
+    """Argument checks run before any HIP call, so they are testable on the CPU."""
+    L = _native.lib()
+    rc = L.gl_pbr_spmm_i32_f32(8, 8, 3, 3, 0, None, None, None, None, None, None, None, None,
+                               None, 1, 1, None, 1, 0, 0, 0, None)
+    assert rc == _native.GL_ERR_INVALID
+    assert "Unsupported block size" in _native.last_error()
+    with pytest.raises(ValueError):
+        _native.check(rc)
+    rc = L.gl_ppr_run_i32_f32(0, 0, 8, 8, 0, *([None] * 8), 1, None, None, None, None, None,
+                              None, None, None, None, None, None, 0, None, None)
+    assert rc == _native.GL_ERR_INVALID
+    assert L.gl_ppr_workspace_bytes(100, 8, 4, 3) > 100 * 4 * 4
+    assert L.gl_ppr_workspace_bytes(-1, 8, 4, 3) == -1
+    assert L.gl_pbr_build_workspace_bytes(100, 8) > 0
+    rc = L.gl_pbr_build_count(10, 10, 9, 9, 1, None, None, None, None, 0, None, None)
+    assert rc == _native.GL_ERR_INVALID and "64-bit code" in _native.last_error()
+
+
+def test_gpu_entry_points_fail_loudly_without_library(monkeypatch):
+    monkeypatch.setattr(_native, "_lib", None)
+    monkeypatch.setattr(_native, "LIB_PATH", "/nonexistent/libgraphlink_hip.so")
+    with pytest.raises(_native.NativeLibraryMissing):
+        _native.lib()

@@ -1,0 +1,97 @@
+"""GPU PBR builder (north-star subsystem a): bit-exact against the CPU restatement of the
+reference builder (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477), in the
+reference's emission order and in canonical order."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import page_rank_amd as graph_link
+from oracle import pbr as opbr
+from tests.helpers import blocky_csr, random_csr
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("block_codes", "block_coords", "block_offsets", "block_data", "remainder_indptr",
+          "remainder_col_ind", "remainder_data")
+
+
+def _assert_same(g, o):
+    assert g.rows == o.rows and g.cols == o.cols
+    assert g.total_nnz == o.total_nnz
+    for f in FIELDS:
+        a = np.asarray(getattr(g, f))
+        b = np.asarray(getattr(o, f))
+        assert a.dtype == b.dtype, (f, a.dtype, b.dtype)
+        assert a.shape == b.shape, (f, a.shape, b.shape)
+        # bit-exact, values included (compare raw bytes)
+        assert a.tobytes() == b.tobytes(), f
+
+
+def _both(A, br, bc, mn, order):
+    g = graph_link.csr_to_pbr(A, br, bc, mn, order=order)
+    o = opbr.csr_matrix_to_pbr(A, br, bc, mn, order=order)
+    _assert_same(g, o)
+    return g
+
+
+@pytest.mark.parametrize("order", ["reference", "canonical"])
+@pytest.mark.parametrize("N,density", [(64, 0.05), (128, 0.15), (1024, 0.05), (1000, 0.01)])
+@pytest.mark.parametrize("block", [(2, 2), (4, 4), (8, 8), (2, 4), (3, 5), (1, 64), (8, 1)])
+@pytest.mark.parametrize("min_nnz", [0, 1, 2, 4])
+def test_builder_random(order, N, density, block, min_nnz, device):
+    A = random_csr(N, density, seed=N + min_nnz)
+    _both(A, block[0], block[1], min_nnz, order)
+
+
+@pytest.mark.parametrize("order", ["reference", "canonical"])
+@pytest.mark.parametrize("dtypes", [(np.int32, np.float32), (np.int64, np.float32),
+                                    (np.int32, np.float64), (np.int64, np.float64)])
+def test_builder_dtypes(order, dtypes, device):
+    it, st = dtypes
+    A = blocky_csr(2003, 8, 3, 1, 40, scatter=5000, seed=5).astype(st)
+    A.indices = A.indices.astype(it)
+    A.indptr = A.indptr.astype(it)
+    g = _both(A, 8, 8, 4, order)
+    assert type(g).__name__ == {(np.int32, np.float32): "PBRMatrixInt32Float",
+                                (np.int64, np.float32): "PBRMatrixInt64Float",
+                                (np.int32, np.float64): "PBRMatrixInt32Double",
+                                (np.int64, np.float64): "PBRMatrixInt64Double"}[dtypes]
+
+
+def test_builder_larger_blocky(device):
+    """A 100k-row block-clustered matrix with hub-like dense tiles and a wide remainder."""
+    A = blocky_csr(100_003, 8, 8, 4, 40, scatter=200_000, seed=9, band=64)
+    _both(A, 8, 8, 4, "reference")
+
+
+def test_builder_edge_cases(device):
+    for A in (sp.csr_matrix((0, 0), dtype=np.float32), sp.csr_matrix((5, 7), dtype=np.float32),
+              sp.csr_matrix(np.ones((9, 9), np.float32)), random_csr(17, 0.3, m=40)):
+        for block in ((2, 2), (8, 8), (3, 3)):
+            for mn in (0, 1, 4, 64, 255):
+                _both(A, block[0], block[1], mn, "reference")
+
+
+def test_builder_hub_stripe(device):
+    """One stripe holding far more distinct blocks than lanes (a hub row of 50k columns)."""
+    n = 60_000
+    rows = np.concatenate([np.full(50_000, 3), np.arange(n)])
+    cols = np.concatenate([np.arange(0, 100_000, 2)[:50_000] % n, np.arange(n)])
+    key = np.unique(rows.astype(np.int64) * n + cols)
+    A = sp.csr_matrix((np.arange(key.size, dtype=np.float32), (key // n, key % n)), shape=(n, n))
+    _both(A, 8, 8, 2, "reference")
+
+
+def test_noncanonical_input_is_canonicalized(device):
+    A = sp.csr_matrix((np.array([1., 2., 3.], np.float32), np.array([2, 0, 2]),
+                       np.array([0, 3, 3])), shape=(2, 4))
+    assert not A.has_canonical_format
+    g = graph_link.csr_to_pbr(A, 2, 2, 1)
+    B = A.copy()
+    B.sum_duplicates()
+    _assert_same(g, opbr.csr_matrix_to_pbr(B, 2, 2, 1))
+
+
+def test_builder_type_errors():
+    with pytest.raises(TypeError):
+        graph_link.csr_to_pbr(np.zeros((3, 3)))

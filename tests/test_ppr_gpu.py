@@ -1,0 +1,164 @@
+"""GPU Personalized PageRank (north-star subsystem d and the persistent loop).
+
+test_ppr_accuracy mirrors Tests/test_ppr_accuracy.py:58-96 (networkx oracle, atol 1e-3,
+mass 1 +- 1e-4); the remaining tests pin the batched path against the FP64 oracle
+(oracle/ppr.py) with tighter tolerances stated per test.
+"""
+import networkx as nx
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import page_rank_amd as graph_link
+from oracle import ppr as oppr
+
+pytestmark = pytest.mark.gpu
+
+
+def _build_graph_and_csr(N, p, seed=42):
+    """Tests/test_ppr_accuracy.py:14-25."""
+    G = nx.erdos_renyi_graph(N, p, directed=True, seed=seed)
+    A_nx = nx.to_scipy_sparse_array(G, nodelist=range(N), format="csr", dtype=np.float32)
+    A_col = sp.csr_matrix(A_nx.T)
+    return G, graph_link.normalize_transition_matrix(A_col)
+
+
+def _nx_ppr(G, N, src, alpha, tol):
+    pers = {n: (1.0 if n == src else 0.0) for n in range(N)}
+    dangling_uniform = {n: 1.0 / N for n in range(N)}
+    scores = nx.pagerank(G, alpha=alpha, personalization=pers, dangling=dangling_uniform,
+                         tol=tol, max_iter=500)
+    return np.array([scores[n] for n in range(N)], dtype=np.float32)
+
+
+@pytest.mark.parametrize("N,p", [(50, 0.10), (100, 0.05)])
+@pytest.mark.parametrize("alpha", [0.85, 0.90])
+@pytest.mark.parametrize("block_size", [2, 4, 8])
+@pytest.mark.parametrize("min_nnz", [1, 4])
+def test_ppr_accuracy(N, p, alpha, block_size, min_nnz, device):
+    G, A_norm = _build_graph_and_csr(N, p)
+    sources = [0, N // 4, N // 2, N - 1]
+    pbr = graph_link.csr_to_pbr(A_norm, block_rows=block_size, block_cols=block_size,
+                                min_nnz_per_block=min_nnz).to(device)
+    X, iters, converged = graph_link.run_personalized_pagerank(
+        pbr, torch.tensor(sources, dtype=torch.int32, device=device), alpha=alpha,
+        max_iterations=500, tolerance=1e-6)
+    assert converged
+    X_cpu = X.cpu().numpy()
+    # the FP64 restatement of the reference loop stops at the same iteration
+    Xr, it_r, conv_r, _ = oppr.reference_loop(A_norm, sources, alpha, 500, 1e-6)
+    assert conv_r and abs(iters - it_r) <= 1
+    for k, src in enumerate(sources):
+        col = X_cpu[:, k]
+        nx_col = _nx_ppr(G, N, src, alpha, tol=1e-6)
+        assert np.isclose(col.sum(), 1.0, atol=1e-4)
+        assert np.allclose(col, nx_col, atol=1e-3)
+        assert np.abs(col - Xr[:, k]).sum() < 1e-5
+
+
+def test_history_and_nonconvergence(device):
+    G, A_norm = _build_graph_and_csr(100, 0.05)
+    pbr = graph_link.csr_to_pbr(A_norm, 4, 4, 1).to(device)
+    src = torch.tensor([1, 7], dtype=torch.int32, device=device)
+    X, iters, conv, hist = graph_link.run_personalized_pagerank(
+        pbr, src, max_iterations=5, tolerance=1e-12, return_history=True)
+    assert iters == 5 and not conv and len(hist) == 5
+    _, _, _, hist_ref = oppr.reference_loop(A_norm, [1, 7], 0.85, 5, 1e-12)
+    assert np.allclose(hist, hist_ref, rtol=1e-4)
+    assert all(hist[i] > hist[i + 1] for i in range(4))
+
+
+def _sbm(n, comm, p_in, p_out, seed):
+    rng = np.random.default_rng(seed)
+    c = np.arange(n) // comm
+    dense = rng.random((n, n))
+    same = c[:, None] == c[None, :]
+    adj = np.where(same, dense < p_in, dense < p_out)
+    np.fill_diagonal(adj, False)
+    return sp.csr_matrix(adj.astype(np.float32))  # adj[source, target]
+
+
+@pytest.mark.parametrize("dangling_mode", ["none", "uniform", "personalization"])
+@pytest.mark.parametrize("retire", ["together", "per_column"])
+@pytest.mark.parametrize("kw", [0, 1, 8, 16, 64])
+@pytest.mark.parametrize("graph_iters", [0, 4])
+def test_batched_seed_sets(dangling_mode, retire, kw, graph_iters, device):
+    """Seed sets of uneven size as columns; dangling rows; FP64 oracle with the same
+    stopping rule; L1 error to the oracle's iterate < 2e-6 per column."""
+    n = 600
+    adj = _sbm(n, 100, 0.08, 0.004, seed=1)
+    adj = adj.tolil()
+    adj[5, :] = 0       # dangling sources
+    adj[333, :] = 0
+    A_col = sp.csr_matrix(adj.T)
+    A, dangling = oppr.column_stochastic(A_col)
+    A32 = sp.csr_matrix(A, dtype=np.float32)
+    rng = np.random.default_rng(3)
+    seed_sets = [rng.choice(n, size=int(rng.integers(1, 9)), replace=False) for _ in range(11)]
+    seed_sets[0] = np.array([5])          # seed on a dangling node
+    seed_sets[1] = np.array([7, 7, 9])    # duplicate seed counts twice
+    pbr = graph_link.csr_to_pbr(A32, 8, 8, 2).to(device)
+    mode = dangling_mode
+    res = graph_link.run_ppr(pbr, seed_sets, alpha=0.85, tol=1e-6, max_iter=200,
+                             dangling=dangling if mode != "none" else None, dangling_mode=mode,
+                             retire=retire, kw=kw, graph_iters=graph_iters)
+    P = oppr.personalization_matrix(n, seed_sets)
+    Xo, it_o, conv_o, _ = oppr.power(A32, P, 0.85, 1e-6, 200, dangling, mode, retire)
+    X = res.X.cpu().numpy()
+    assert res.converged.all() and conv_o.all()
+    assert np.all(np.abs(res.iterations - it_o) <= 1)
+    err = np.abs(X - Xo).sum(axis=0)
+    assert err.max() < 2e-6, err
+    if mode != "none":  # with dangling mass returned, every column stays a distribution
+        assert np.allclose(X.sum(axis=0), 1.0, atol=1e-5)
+
+
+def test_warm_start_from_solution_converges_immediately(device):
+    n = 400
+    adj = _sbm(n, 50, 0.1, 0.005, seed=4)
+    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    A32 = sp.csr_matrix(A, dtype=np.float32)
+    pbr = graph_link.csr_to_pbr(A32, 8, 8, 2).to(device)
+    seed_sets = [[1], [2, 3], [10, 20, 30]]
+    P = oppr.personalization_matrix(n, seed_sets)
+    exact = np.stack([oppr.solve(A32, P[:, k], 0.85, dangling, "uniform") for k in range(3)], 1)
+    res = graph_link.run_ppr(pbr, seed_sets, tol=1e-6, dangling=dangling,
+                             dangling_mode="uniform", x0=exact.astype(np.float32))
+    assert res.total_iterations <= 2
+    cold = graph_link.run_ppr(pbr, seed_sets, tol=1e-6, dangling=dangling, dangling_mode="uniform")
+    assert cold.total_iterations > 30
+    assert np.abs(res.X.cpu().numpy() - exact).sum(axis=0).max() < 1e-5
+
+
+def test_closed_forms(device):
+    # alpha = 0: the result is the personalization itself after one step
+    A = sp.csr_matrix(np.full((5, 5), 0.2, np.float32))
+    pbr = graph_link.csr_to_pbr(A, 2, 2, 1).to(device)
+    r = graph_link.run_ppr(pbr, [[3], [0, 1]], alpha=0.0, tol=1e-7)
+    X = r.X.cpu().numpy()
+    assert np.allclose(X[:, 0], [0, 0, 0, 1, 0]) and np.allclose(X[:, 1], [.5, .5, 0, 0, 0])
+    # complete graph K5 with uniform personalization: 1/5 everywhere
+    Ak = sp.csr_matrix((np.ones((5, 5)) - np.eye(5)).astype(np.float32) / 4)
+    pk = graph_link.csr_to_pbr(Ak, 4, 4, 1).to(device)
+    r = graph_link.run_ppr(pk, [[0, 1, 2, 3, 4]], tol=1e-8, max_iter=500)
+    assert np.allclose(r.X.cpu().numpy()[:, 0], 0.2, atol=1e-6)
+    # isolated seeded node with personalization-dangling: all mass stays on it
+    Ai = sp.csr_matrix((4, 4), dtype=np.float32)
+    pi = graph_link.csr_to_pbr(Ai, 2, 2, 1).to(device)
+    r = graph_link.run_ppr(pi, [[2]], dangling=np.ones(4, bool), dangling_mode="personalization")
+    assert np.allclose(r.X.cpu().numpy()[:, 0], [0, 0, 1, 0], atol=1e-7)
+
+
+def test_errors(device):
+    A = sp.csr_matrix(np.full((4, 4), 0.25, np.float32))
+    pbr = graph_link.csr_to_pbr(A, 2, 2, 1).to(device)
+    with pytest.raises(ValueError):
+        graph_link.run_ppr(pbr, [[9]])
+    with pytest.raises(ValueError):
+        graph_link.run_ppr(pbr, [[]])
+    with pytest.raises(ValueError):
+        graph_link.run_ppr(pbr, [[1]], dangling_mode="uniform")
+    cpu = pbr.to("cpu")
+    with pytest.raises(RuntimeError):
+        graph_link.run_personalized_pagerank(cpu, torch.tensor([0], dtype=torch.int32))
