@@ -152,3 +152,16 @@ def bucket_counts(hint, keys):
     fn.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
     final = fn(int(hint), _p(keys), keys.size, _p(counts))
     return int(final), counts
+
+
+def bucket_trace(hint, keys, stripe_ptr, key_dtype=np.int32):
+    """Bucket count after each emplace, stripes separated by clear()."""
+    keys = np.ascontiguousarray(keys, dtype=key_dtype)
+    sptr = np.ascontiguousarray(stripe_ptr, dtype=np.int64)
+    counts = np.zeros(keys.size, dtype=np.int64)
+    suf = "i32" if np.dtype(key_dtype) == np.int32 else "i64"
+    fn = getattr(lib(), f"orc_bucket_trace_{suf}")
+    fn.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                   ctypes.c_void_p]
+    fn(int(hint), _p(keys), _p(sptr), sptr.size - 1, _p(counts))
+    return counts

@@ -256,3 +256,27 @@ extern "C" int64_t orc_bucket_count_after(int64_t hint, const int32_t* keys, int
   }
   return (int64_t)m.bucket_count();
 }
+
+// Bucket count after every emplace when the keys of consecutive stripes are inserted into one
+// std::unordered_map<I, int>(hint) that is clear()ed between stripes (the reference builder's
+// sequence of container operations, pbr_matrix.cpp:374-428).
+template <typename I>
+static void bucket_trace(int64_t hint, const I* keys, const int64_t* sptr, int64_t ns,
+                         int64_t* counts) {
+  std::unordered_map<I, int> m((size_t)hint);
+  for (int64_t s = 0; s < ns; ++s) {
+    for (int64_t i = sptr[s]; i < sptr[s + 1]; ++i) {
+      m.emplace(keys[i], 0);
+      counts[i] = (int64_t)m.bucket_count();
+    }
+    m.clear();
+  }
+}
+extern "C" void orc_bucket_trace_i32(int64_t hint, const int32_t* keys, const int64_t* sptr,
+                                     int64_t ns, int64_t* counts) {
+  bucket_trace<int32_t>(hint, keys, sptr, ns, counts);
+}
+extern "C" void orc_bucket_trace_i64(int64_t hint, const int64_t* keys, const int64_t* sptr,
+                                     int64_t ns, int64_t* counts) {
+  bucket_trace<int64_t>(hint, keys, sptr, ns, counts);
+}

@@ -73,3 +73,33 @@ def pbr_matmul(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0):
 def pbr_batched_matmul_cpu(pbr_mat, x):
     raise NotImplementedError("pbr_batched_matmul_cpu: this build has no CPU SpMM; use "
                               "pbr_matmul with a GPU tensor")
+
+
+class SpmmPlan:
+    """A bound SpMM call (matrix, x, y, mode, kw fixed): ``run()`` costs one C call.
+    For launch-bound loops (benchmarks, repeated products on the same buffers)."""
+
+    def __init__(self, pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False,
+                 mode="fused", kw=0):
+        meta = _meta_or_raise(pbr_mat)
+        if x.dtype != torch.float32 or y.dtype != torch.float32:
+            raise TypeError("SpmmPlan needs float32 x and y")
+        if x.dim() != 2 or x.shape[0] != pbr_mat.cols or y.shape != (pbr_mat.rows, x.shape[1]):
+            raise ValueError("shape mismatch")
+        if x.stride(1) != 1 or y.stride(1) != 1:
+            raise ValueError("x and y must have unit stride along K")
+        self._keep = (meta, x, y)
+        self._fn = _native.lib().gl_pbr_spmm_i32_f32
+        m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
+        self._args = (
+            pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
+            meta["codes"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+            meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
+            meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
+            meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), x.shape[1], y.data_ptr(),
+            y.stride(0), 1 if accumulate else 0, m, kw, _native.stream_handle(x.device))
+
+    def run(self):
+        rc = self._fn(*self._args)
+        if rc:
+            _native.check(rc, "gl_pbr_spmm_i32_f32")

@@ -127,7 +127,7 @@ def test_warm_start_from_solution_converges_immediately(device):
                              dangling_mode="uniform", x0=exact.astype(np.float32))
     assert res.total_iterations <= 2
     cold = graph_link.run_ppr(pbr, seed_sets, tol=1e-6, dangling=dangling, dangling_mode="uniform")
-    assert cold.total_iterations > 30
+    assert cold.total_iterations >= 20 and cold.converged.all()
     assert np.abs(res.X.cpu().numpy() - exact).sum(axis=0).max() < 1e-5
 
 
