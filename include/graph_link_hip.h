@@ -57,13 +57,15 @@ int gl_pbr_stripe_index(int64_t rows, int32_t block_rows, int64_t num_tiles,
  *   mode GL_SPMM_FUSED : one stripe-owner kernel does tiles + remainder, plain stores.
  *   mode GL_SPMM_SPLIT : block kernel (b) on `stream`, CSR-remainder kernel (c) on the
  *                        library's side stream, joined back to `stream` with events.
+ * num_vals = offsets[num_tiles] (tile values), num_rem = rem_indptr[rows] (remainder nnz).
  * kw: RHS columns per launch (1,2,4,8 bit-lane kernel; 16,32,64 feature-lane kernel);
  *     0 lets the library choose. Block sizes 2x2, 4x4, 8x8 (as the reference).
  * ---------------------------------------------------------------------------------- */
 #define GL_SPMM_FUSED 0
 #define GL_SPMM_SPLIT 1
 int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                        int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                        int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                        const uint64_t* codes, const int32_t* coords,
                         const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
                         const int32_t* rem_indptr, const int32_t* rem_col_ind,
                         const float* rem_vals, const float* X, int64_t ldx, int64_t K,
@@ -72,7 +74,8 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
 
 /* Block-tile part alone (north-star subsystem b). Rows with no tile get 0 (or keep Y). */
 int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
-                              int32_t block_cols, int64_t num_tiles, const uint64_t* codes,
+                              int32_t block_cols, int64_t num_tiles, int64_t num_vals,
+                              const uint64_t* codes,
                               const int32_t* coords, const int32_t* offsets, const float* data,
                               const int32_t* stripe_ptr, const float* X, int64_t ldx,
                               int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
@@ -80,7 +83,7 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
 
 /* CSR SpMM alone (north-star subsystem c; the PBR remainder, or any CSR matrix).
  * Replaces launch_csr_spmm (Backend/graph_link/kernels/pbr_kernel.cu:404-478). */
-int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, const int32_t* indptr,
+int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* indptr,
                         const int32_t* col_ind, const float* vals, const float* X, int64_t ldx,
                         int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
                         void* stream);
@@ -128,7 +131,8 @@ int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,
                                int64_t num_seed_entries);
 
 int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                       int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                       int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                       const uint64_t* codes, const int32_t* coords,
                        const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
                        const int32_t* rem_indptr, const int32_t* rem_col_ind,
                        const float* rem_vals, int64_t K, const int32_t* seed_ptr,

@@ -51,16 +51,20 @@ _PROTOS = {
     "gl_abi_version": (c_i32, []),
     "gl_device_info": (ctypes.c_int, [c_p, c_p, c_p, c_p]),
     "gl_pbr_stripe_index": (ctypes.c_int, [c_i64, c_i32, c_i64, c_p, c_p, c_p]),
-    "gl_pbr_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_p, c_p, c_p, c_p,
+    "gl_pbr_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64, c_i64,
+                                           c_p, c_p, c_p, c_p,
                                            c_p, c_p, c_p, c_p, c_p, c_i64, c_i64, c_p, c_i64,
                                            c_i32, c_i32, c_i32, c_p]),
-    "gl_pbr_block_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_p, c_p,
+    "gl_pbr_block_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64,
+                                                 c_p, c_p,
                                                  c_p, c_p, c_p, c_p, c_i64, c_i64, c_p, c_i64,
                                                  c_i32, c_i32, c_p]),
-    "gl_csr_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_i64, c_i64, c_p,
+    "gl_csr_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_i64, c_i64,
+                                           c_p,
                                            c_i64, c_i32, c_i32, c_p]),
     "gl_ppr_workspace_bytes": (c_i64, [c_i64, c_i32, c_i64, c_i64]),
-    "gl_ppr_run_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_p, c_p, c_p, c_p,
+    "gl_ppr_run_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64, c_i64,
+                                          c_p, c_p, c_p, c_p,
                                           c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
                                           ctypes.POINTER(GlOpts), c_p, c_p, c_p, c_p, c_p,
                                           c_i64, c_p, c_p]),

@@ -114,7 +114,7 @@ extern "C" {
 
 const char* gl_last_error(void) { return t_last_error.c_str(); }
 
-int32_t gl_abi_version(void) { return 1; }
+int32_t gl_abi_version(void) { return 2; }
 
 int gl_device_info(int32_t* num_cus, int32_t* lds_bytes_per_cu, int32_t* clock_khz,
                    int64_t* hbm_bytes) {
@@ -163,7 +163,8 @@ static int check_dense(int64_t rows, int64_t cols, const float* X, int64_t ldx, 
 }
 
 int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
-                              int32_t block_cols, int64_t num_tiles, const uint64_t* codes,
+                              int32_t block_cols, int64_t num_tiles, int64_t num_vals,
+                              const uint64_t* codes,
                               const int32_t* coords, const int32_t* offsets, const float* data,
                               const int32_t* stripe_ptr, const float* X, int64_t ldx,
                               int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
@@ -175,12 +176,12 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
   if (kw == 0) kw = choose_kw(K);
   GL_REQUIRE(valid_kw(kw), "kw must be one of 1,2,4,8,16,32,64");
   if (rows == 0) return GL_OK;
-  PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, codes, coords, offsets, data,
-            stripe_ptr, nullptr, nullptr, nullptr};
+  PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, 0,
+            codes, coords, offsets, data, stripe_ptr, nullptr, nullptr, nullptr};
   const int nwg = grid_for(m.num_stripes, 0);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
-    RhsView d{X, ldx, K, k0};
-    EpiStore e{Y, ldy, accumulate};
+    RhsView d{X, ldx, K, k0, cols * ldx};
+    EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
     if ((rc = launch_stripe<EpiStore, true, false>(m, d, block_rows, kw, e, nwg,
                                                    (hipStream_t)stream)))
       return rc;
@@ -188,7 +189,7 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
   return GL_OK;
 }
 
-int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, const int32_t* indptr,
+int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* indptr,
                         const int32_t* col_ind, const float* vals, const float* X, int64_t ldx,
                         int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
                         void* stream) {
@@ -200,12 +201,12 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, const int32_t* indptr,
   GL_REQUIRE(valid_kw(kw), "kw must be one of 1,2,4,8,16,32,64");
   if (rows == 0) return GL_OK;
   // Row groups of 8 rows; one wave owns a group (the kernel's "stripe").
-  PbrView m{rows, cols, (rows + 7) / 8, nullptr, nullptr, nullptr, nullptr, nullptr,
+  PbrView m{rows, cols, (rows + 7) / 8, 0, 0, nnz, nullptr, nullptr, nullptr, nullptr, nullptr,
             indptr, col_ind, vals};
   const int nwg = grid_for(m.num_stripes, 0);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
-    RhsView d{X, ldx, K, k0};
-    EpiStore e{Y, ldy, accumulate};
+    RhsView d{X, ldx, K, k0, cols * ldx};
+    EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
     if ((rc = launch_stripe<EpiStore, false, true>(m, d, 8, kw, e, nwg, (hipStream_t)stream)))
       return rc;
   }
@@ -213,7 +214,8 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, const int32_t* indptr,
 }
 
 int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                        int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                        int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                        const uint64_t* codes, const int32_t* coords,
                         const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
                         const int32_t* rem_indptr, const int32_t* rem_col_ind,
                         const float* rem_vals, const float* X, int64_t ldx, int64_t K,
@@ -230,28 +232,28 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
   if (rows == 0) return GL_OK;
   hipStream_t st = (hipStream_t)stream;
   if (mode == GL_SPMM_SPLIT) {
-    rc = gl_pbr_block_spmm_i32_f32(rows, cols, block_rows, block_cols, num_tiles, codes, coords,
-                                   offsets, data, stripe_ptr, X, ldx, K, Y, ldy, accumulate, kw,
-                                   stream);
+    rc = gl_pbr_block_spmm_i32_f32(rows, cols, block_rows, block_cols, num_tiles, num_vals,
+                                   codes, coords, offsets, data, stripe_ptr, X, ldx, K, Y, ldy,
+                                   accumulate, kw, stream);
     if (rc) return rc;
     // the remainder kernel runs on the library's side stream, ordered after the block
     // kernel (both own whole rows of Y, so the second one adds without atomics)
     hipStream_t side = side_stream();
     GL_HIP(hipEventRecord(lib_event(0), st));
     GL_HIP(hipStreamWaitEvent(side, lib_event(0), 0));
-    rc = gl_csr_spmm_i32_f32(rows, cols, rem_indptr, rem_col_ind, rem_vals, X, ldx, K, Y, ldy,
-                             1, kw, side);
+    rc = gl_csr_spmm_i32_f32(rows, cols, num_rem, rem_indptr, rem_col_ind, rem_vals, X, ldx, K,
+                             Y, ldy, 1, kw, side);
     if (rc) return rc;
     GL_HIP(hipEventRecord(lib_event(1), side));
     GL_HIP(hipStreamWaitEvent(st, lib_event(1), 0));
     return GL_OK;
   }
-  PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, codes, coords, offsets, data,
-            stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
+  PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, num_rem,
+            codes, coords, offsets, data, stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
   const int nwg = grid_for(m.num_stripes, 0);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
-    RhsView d{X, ldx, K, k0};
-    EpiStore e{Y, ldy, accumulate};
+    RhsView d{X, ldx, K, k0, cols * ldx};
+    EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
     if ((rc = launch_stripe<EpiStore, true, true>(m, d, block_rows, kw, e, nwg, st))) return rc;
   }
   return GL_OK;

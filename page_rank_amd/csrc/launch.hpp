@@ -13,24 +13,37 @@ int grid_for(int64_t work_items, int wg_per_cu);
 int choose_kw(int64_t K);
 bool valid_kw(int kw);
 
-template <int B, class Epi, bool TILES, bool REM>
-int launch_stripe_b(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
-                    hipStream_t st) {
+template <int B, class Epi, bool TILES, bool REM, bool BUF>
+int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
+                     hipStream_t st) {
   const dim3 grid(nwg), block(256);
   switch (kw) {
-    case 1: hipLaunchKernelGGL((k_stripe_bit<B, 1, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 2: hipLaunchKernelGGL((k_stripe_bit<B, 2, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 4: hipLaunchKernelGGL((k_stripe_bit<B, 4, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 8: hipLaunchKernelGGL((k_stripe_bit<B, 8, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 16: hipLaunchKernelGGL((k_stripe_feat<B, 16, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 32: hipLaunchKernelGGL((k_stripe_feat<B, 32, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
-    case 64: hipLaunchKernelGGL((k_stripe_feat<B, 64, TILES, REM, Epi>), grid, block, 0, st, m, d, e); break;
+#define GL_K(KW, KER) \
+    case KW: hipLaunchKernelGGL((KER<B, KW, TILES, REM, BUF, Epi>), grid, block, 0, st, m, d, e); break;
+    GL_K(1, k_stripe_bit) GL_K(2, k_stripe_bit) GL_K(4, k_stripe_bit) GL_K(8, k_stripe_bit)
+    GL_K(16, k_stripe_feat) GL_K(32, k_stripe_feat) GL_K(64, k_stripe_feat)
+#undef GL_K
     default:
       set_error("unsupported kw %d", kw);
       return GL_ERR_INVALID;
   }
   GL_LAUNCH_CHECK("k_stripe");
   return GL_OK;
+}
+
+// Buffer-descriptor loads when every array of the product is below 2 GB and X rows
+// are addressable with 24-bit products.
+inline bool buffer_ok(const PbrView& m, const RhsView& d) {
+  return m.num_tiles * 8 < kBufLimit && m.num_vals * 4 < kBufLimit &&
+         m.num_rem * 4 < kBufLimit && (m.rows + 1) * 4 < kBufLimit && d.len * 4 < kBufLimit &&
+         m.cols < (1 << 24) && d.ldx < (1 << 24);
+}
+
+template <int B, class Epi, bool TILES, bool REM>
+int launch_stripe_b(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
+                    hipStream_t st) {
+  if (buffer_ok(m, d)) return launch_stripe_bb<B, Epi, TILES, REM, true>(m, d, kw, e, nwg, st);
+  return launch_stripe_bb<B, Epi, TILES, REM, false>(m, d, kw, e, nwg, st);
 }
 
 template <class Epi, bool TILES, bool REM>

@@ -225,7 +225,8 @@ int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,
 }
 
 int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                       int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                       int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                       const uint64_t* codes, const int32_t* coords,
                        const int32_t* offsets, const float* data, const int32_t* stripe_ptr,
                        const int32_t* rem_indptr, const int32_t* rem_col_ind,
                        const float* rem_vals, int64_t K, const int32_t* seed_ptr,
@@ -334,8 +335,8 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
     GL_LAUNCH_CHECK("k_fill");
   }
 
-  PbrView m{rows, cols, ns, codes, coords, offsets, data, stripe_ptr, rem_indptr, rem_col_ind,
-            rem_vals};
+  PbrView m{rows, cols, ns, num_tiles, num_vals, num_rem, codes, coords, offsets, data,
+            stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
   int nwg = grid_for(ns, opts->grid_mult);
   if (nwg > kMaxWg) nwg = kMaxWg - (kMaxWg % kNumXcd);
   const int together = opts->retire_mode == GL_RETIRE_TOGETHER ? 1 : 0;
@@ -345,11 +346,11 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
     const float* xc = bufs[parity];
     float* xn = bufs[parity ^ 1];
     for (int64_t k0 = 0; k0 < K; k0 += kw) {
-      RhsView d{xc, K, K, k0};
+      RhsView d{xc, K, K, k0, rows * K};
       EpiPpr e{xc, xn, K, opts->alpha, w.cu, w.tele, w.frozen,
                opts->dangling_mode != GL_DANGLING_NONE ? dangling_mask : nullptr,
                w.sptr, w.srow, w.scol, w.sw, w.ctl, w.part_res, w.part_dang, K, k0,
-               0.f, 0.f, 0, 0};
+               fits_u24(rows, K), 0.f, 0.f, 0, 0};
       int rc = launch_stripe<EpiPpr, true, true>(m, d, block_rows, kw, e, nwg, st);
       if (rc) return rc;
     }

@@ -25,12 +25,21 @@
 #ifndef GL_FUSED_DPP_FMA
 #define GL_FUSED_DPP_FMA 1
 #endif
+// tile groups whose loads are issued together (feature-lane / bit-lane kernels)
+#ifndef GL_FEAT_TILE_U
+#define GL_FEAT_TILE_U 2
+#endif
+#ifndef GL_BIT_TILE_U
+#define GL_BIT_TILE_U 4
+#endif
+
 
 namespace gl {
 
 struct PbrView {
   int64_t rows, cols;
   int64_t num_stripes;
+  int64_t num_tiles, num_vals, num_rem;  // array lengths (buffer-descriptor bounds)
   const uint64_t* __restrict__ codes;
   const int32_t* __restrict__ coords;
   const int32_t* __restrict__ offsets;
@@ -46,6 +55,75 @@ struct RhsView {
   int64_t ldx;
   int64_t K;   // total RHS columns
   int64_t k0;  // first column of this launch
+  int64_t len; // elements of x (rows of X times ldx)
+};
+
+// --------------------------------------------------------------------------------------
+// Predicated loads. Ld<true> goes through a raw buffer descriptor: a lane whose predicate
+// is false reads an offset beyond the descriptor's bound and gets 0 from the hardware range
+// check, so no branch guards the load and all 32-bit offsets stay in VGPRs (X column steps
+// ride in the scalar offset). Ld<false> is the plain-pointer form for arrays of 2 GB or
+// more. Both return 0 for a false predicate.
+// --------------------------------------------------------------------------------------
+constexpr uint32_t kOob = 0x80000000u;        // never inside a descriptor (< 2 GB)
+constexpr int64_t kBufLimit = 0x7fff0000ll;    // largest array (bytes) for Ld<true>
+
+template <bool BUF>
+struct Ld;
+
+template <>
+struct Ld<true> {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Ld(const void* p, int64_t bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                            0x00020000)) {}
+  __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, ok ? (uint32_t)i * 4u : kOob, sb, 0);
+  }
+  __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
+    return __builtin_bit_cast(float, u32(i, ok, sb));
+  }
+  // Wide loads are built from dword loads at consecutive offsets (the backend merges them
+  // into buffer_load_dwordx2/x4); the vector-result buffer builtins are not used.
+  __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
+    const uint32_t o = ok ? (uint32_t)i * 8u : kOob;
+    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0);
+    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0);
+    return ((uint64_t)hi << 32) | lo;
+  }
+  __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
+    const uint32_t o = ok ? (uint32_t)i * 4u : kOob;
+    return make_float4(
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 8u, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 12u, 0, 0)));
+  }
+};
+
+template <>
+struct Ld<false> {
+  const char* p;
+  __device__ __forceinline__ Ld(const void* q, int64_t) : p((const char*)q) {}
+  __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
+    return ok ? *(const uint32_t*)(p + i * 4 + sb) : 0u;
+  }
+  __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
+    return ok ? *(const float*)(p + i * 4 + sb) : 0.f;
+  }
+  __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
+    return ok ? *(const uint64_t*)(p + i * 8) : 0ull;
+  }
+  __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
+    return ok ? *(const float4*)(p + i * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+};
+
+// The arrays of one SpMM as predicated loaders.
+template <bool BUF>
+struct PbrLd {
+  Ld<BUF> codes, coords, offsets, data, rptr, rcol, rval, x;
+  __device__ __forceinline__ PbrLd(const struct PbrView& m, const RhsView& d);
 };
 
 // --------------------------------------------------------------------------------------
@@ -136,14 +214,24 @@ struct TileFma<2> {
 // once per output element. begin_stripe() runs (wave-uniform) before the stripe's puts and
 // finish() once per lane at the end with the lane's column slot (or -1).
 // --------------------------------------------------------------------------------------
+// Element offset row * ld + col; `u24` (set by the host when rows and ld are below 2^24 and
+// their product below 2^32) selects a full-rate 24-bit multiply.
+__device__ __forceinline__ int64_t elem(int64_t row, int64_t ld, int64_t col, bool u24) {
+  return u24 ? (int64_t)__umul24((uint32_t)row, (uint32_t)ld) + col : row * ld + col;
+}
+inline bool fits_u24(int64_t rows, int64_t ld) {
+  return rows < (1 << 24) && ld < (1 << 24) && rows * ld < (int64_t)1 << 32;
+}
+
 struct EpiStore {
   float* __restrict__ y;
   int64_t ldy;
   int32_t accumulate;
+  bool u24;
   __device__ __forceinline__ bool active() const { return true; }
   __device__ __forceinline__ void begin_stripe(int64_t) {}
   __device__ __forceinline__ void put(int64_t row, int64_t col, float acc) {
-    float* p = y + row * ldy + col;
+    float* p = y + elem(row, ldy, col, u24);
     *p = accumulate ? (*p + acc) : acc;
   }
   __device__ __forceinline__ void finish(int, float*) {}
@@ -182,6 +270,7 @@ struct EpiPpr {
   float* __restrict__ part_dang;  // [gridDim.x][K]
   int64_t K;
   int64_t k0;
+  bool u24;
   // per-lane state
   float res;
   float dang;
@@ -193,7 +282,7 @@ struct EpiPpr {
     e1 = sptr[s + 1];
   }
   __device__ __forceinline__ void put(int64_t row, int64_t col, float acc) {
-    const int64_t ix = row * ld + col;
+    const int64_t ix = elem(row, ld, col, u24);
     const float xc = xcur[ix];
     float xn = xc;
     if (!frozen[col]) {
@@ -227,19 +316,146 @@ struct EpiPpr {
 };
 
 // --------------------------------------------------------------------------------------
-// Feature-lane kernel, KW in {16, 32, 64}.
+// Cross-lane sums. DPP for neighbours inside a 16-lane row, bpermute for wider strides.
 // --------------------------------------------------------------------------------------
-template <int B, int KW, bool TILES, bool REM, class Epi>
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+// Sum over the lanes that differ only in bit `MASK` of the lane id.
+template <int MASK>
+__device__ __forceinline__ float xor_sum(float v) {
+  if constexpr (MASK == 1) return v + dpp<0xB1>(v);  // quad_perm [1,0,3,2]
+  else if constexpr (MASK == 2) return v + dpp<0x4E>(v);  // quad_perm [2,3,0,1]
+  else return v + __shfl_xor(v, MASK, kWave);
+}
+// Sum over aligned groups of 8 lanes (row_half_mirror completes the 8-lane sum once the
+// two quads are uniform).
+__device__ __forceinline__ float sum8(float v) {
+  v = xor_sum<1>(v);
+  v = xor_sum<2>(v);
+  return v + dpp<0x141>(v);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool BUF>
+__device__ __forceinline__ PbrLd<BUF>::PbrLd(const PbrView& m, const RhsView& d)
+    : codes(m.codes, m.num_tiles * 8),
+      coords(m.coords, m.num_tiles * 8),
+      offsets(m.offsets, (m.num_tiles + 1) * 4),
+      data(m.data, m.num_vals * 4),
+      rptr(m.rem_ptr, (m.rows + 1) * 4),
+      rcol(m.rem_col, m.num_rem * 4),
+      rval(m.rem_val, m.num_rem * 4),
+      x(d.x + d.k0, (d.len - d.k0) * 4) {}
+
+// N (1..8) consecutive floats of X starting at element i, zeros for lanes with valid <= 0.
+// `wide` (wave-uniform: 16-byte aligned rows and a full column chunk) selects float4 loads.
+template <int N, bool BUF>
+__device__ __forceinline__ void load_row(const Ld<BUF>& xl, int64_t i, bool wide, int valid,
+                                         float (&out)[N]) {
+  if constexpr (N >= 4) {
+    if (wide) {
+#pragma unroll
+      for (int k = 0; k < N; k += 4) {
+        const float4 t = xl.f32x4(i + k, valid > 0);
+        out[k] = t.x;
+        out[k + 1] = t.y;
+        out[k + 2] = t.z;
+        out[k + 3] = t.w;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) out[k] = xl.f32(i + k, k < valid);
+}
+
+// Bit p of a tile code given as two 32-bit words, and the number of set bits below it
+// (the index of p's value inside the tile's packed values); 32-bit ALU only.
+__device__ __forceinline__ bool code_bit(uint32_t clo, uint32_t chi, int p) {
+  const uint32_t w = p < 32 ? clo : chi;
+  return (w >> (p & 31)) & 1u;
+}
+__device__ __forceinline__ int code_prefix(uint32_t clo, uint32_t chi, int p) {
+  const uint32_t w = p < 32 ? clo : chi;
+  const int base = p < 32 ? 0 : __popc(clo);
+  return base + __popc(w & ((1u << (p & 31)) - 1u));
+}
+// Element offset of row r of X: 24-bit multiply on the buffer path (rows and ldx below
+// 2^24 there), 64-bit otherwise.
+template <bool BUF>
+__device__ __forceinline__ int64_t xrow(int64_t r, int64_t ldx) {
+  if constexpr (BUF) return (int64_t)__umul24((uint32_t)r, (uint32_t)ldx);
+  else return r * ldx;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = max(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
+// Metadata of up to 64 tiles [t, t1), one tile per lane.
+struct TileMeta {
+  uint32_t clo, chi;
+  int c0, off;
+};
+template <bool BUF>
+__device__ __forceinline__ TileMeta load_tile_meta(const PbrLd<BUF>& ld, int t, int t1,
+                                                   int lane) {
+  const int tl = t + lane;
+  const bool ok = tl < t1;
+  const uint64_t c = ld.codes.u64(tl, ok);
+  TileMeta r;
+  r.clo = (uint32_t)c;
+  r.chi = (uint32_t)(c >> 32);
+  r.c0 = (int)ld.coords.u32(2 * (int64_t)tl + 1, ok);
+  r.off = (int)ld.offsets.u32(tl, ok);
+  return r;
+}
+
+// Remainder entries of one matrix row: [j0, j0 + n).
+struct RowSpan {
+  int j0, n;
+};
+template <bool BUF>
+__device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const PbrView& m,
+                                                 int64_t row) {
+  const bool ok = row < m.rows;
+  RowSpan r;
+  r.j0 = (int)ld.rptr.u32(row, ok);
+  r.n = (int)ld.rptr.u32(row + 1, ok) - r.j0;
+  return r;
+}
+
+// --------------------------------------------------------------------------------------
+// Both kernels walk their stripes with a software pipeline: stripe pointers are loaded two
+// stripes ahead, tile metadata and remainder row spans one stripe ahead, and the first
+// remainder entries of a stripe are requested before its tile loads, so each stripe costs
+// about two dependent memory round trips.
+// --------------------------------------------------------------------------------------
+
+// Feature-lane kernel, KW in {16, 32, 64}.
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
 __global__ void __launch_bounds__(256) k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int G = kWave / KW;           // tiles in flight per wave
-  constexpr int P = B * B;                 // bits per tile
-  constexpr int NV = (P + 15) / 16;        // value registers per lane
-  constexpr int L = kWave / B;             // lanes per row in the remainder phase
-  constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // RHS columns per lane there
+  constexpr int P = B * B;                // bits per tile
+  constexpr int NV = (P + 15) / 16;       // value registers per lane
+  constexpr int U = GL_FEAT_TILE_U;       // tile groups whose loads are issued together
+  constexpr int L = kWave / B;            // remainder: lanes per row
+  constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // remainder: RHS columns per lane
+  constexpr int EB = L < 4 ? L : 4;       // remainder: entries whose X loads go together
   __shared__ float red[128];
-  __shared__ float trans[4][B][KW];        // remainder row-layout -> feature-layout
+  __shared__ float racc[4][B * KW];
 
   if (!epi.active()) return;
+  const PbrLd<BUF> ld(m, d);
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const int g = lane / KW;
@@ -247,109 +463,143 @@ __global__ void __launch_bounds__(256) k_stripe_feat(PbrView m, RhsView d, Epi e
   const int o = lane & 15;
   const int64_t colk = d.k0 + f;
   const bool fok = colk < d.K;
-  const float* __restrict__ X = d.x + d.k0;
-  // remainder-phase layout: lane = (row rr, slot q), columns q*FPL .. q*FPL+FPL-1
-  const int rr = lane / L;
-  const int q = lane % L;
-  const bool qok = q * FPL < KW;
+  const int kvalid = (int)min<int64_t>(KW, d.K - d.k0);
+  const bool vec = (d.ldx & 3) == 0 && kvalid == KW;  // float4 rows, full column chunk
+  const uint32_t ldb = (uint32_t)d.ldx * 4u;  // bytes per X row
+  const int rr = lane / L;                // remainder: row of this lane
+  const int qq = lane % L;                // remainder: slot in the row
+  const int kq = qq * FPL;
+  const int nvq = kvalid - kq;
+  const int gbase = lane & ~(L - 1);
+  float* __restrict__ ra = racc[wid];
 
   const Sched sc = xcd_sched(m.num_stripes);
-  for (int64_t s = sc.first; s < sc.end; s += sc.stride) {
+  int64_t s = sc.first;
+  int ta0 = 0, ta1 = 0, tb0 = 0, tb1 = 0;
+  if (TILES && s < sc.end) {
+    ta0 = m.stripe_ptr[s];
+    ta1 = m.stripe_ptr[s + 1];
+  }
+  if (TILES && s + sc.stride < sc.end) {
+    tb0 = m.stripe_ptr[s + sc.stride];
+    tb1 = m.stripe_ptr[s + sc.stride + 1];
+  }
+  TileMeta mt{0u, 0u, 0, 0};
+  if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
+  RowSpan rs{0, 0};
+  if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rr);
+
+  for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
+    const int64_t sn = s + sc.stride;
+    const int64_t snn = sn + sc.stride;
+    // ---- prefetch for the next stripe
+    TileMeta mtn{0u, 0u, 0, 0};
+    int tc0 = 0, tc1 = 0;
+    RowSpan rsn{0, 0};
+    if (TILES) mtn = load_tile_meta(ld, tb0, tb1, lane);
+    if (TILES && snn < sc.end) {
+      tc0 = m.stripe_ptr[snn];
+      tc1 = m.stripe_ptr[snn + 1];
+    }
+    if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rr);
+    // ---- first remainder entries of this stripe
+    int scol = 0;
+    float sval = 0.f;
+    if constexpr (REM) {
+      const bool ok = qq < rs.n;
+      scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
+      sval = ld.rval.f32(rs.j0 + qq, ok);
+    }
+
     float acc[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) acc[r] = 0.f;
 
     if constexpr (TILES) {
-      const int t0 = m.stripe_ptr[s];
-      const int t1 = m.stripe_ptr[s + 1];
-      for (int tc = t0; tc < t1; tc += kWave) {
-        // stage up to 64 tiles' metadata, one tile per lane
-        const int tl = tc + lane;
-        uint64_t mcode = 0;
-        int mc0 = 0, moff = 0;
-        if (tl < t1) {
-          mcode = m.codes[tl];
-          mc0 = m.coords[2 * (int64_t)tl + 1];
-          moff = m.offsets[tl];
-        }
-        const int tce = min(t1, tc + kWave);
-
-        for (int tb = tc; tb < tce; tb += G) {
-          const int src = tb - tc + g;  // lane holding this group's tile metadata
-          const bool tv = tb + g < tce;
-          const uint32_t clo = __shfl((uint32_t)mcode, src, kWave);
-          const uint32_t chi = __shfl((uint32_t)(mcode >> 32), src, kWave);
-          const int c0 = __shfl(mc0, src, kWave);
-          const int off = __shfl(moff, src, kWave);
-          const uint64_t code = tv ? (((uint64_t)chi << 32) | clo) : 0ull;
-          float v[NV];
+      for (int tc = ta0; tc < ta1; tc += kWave) {
+        if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
+        const int tce = min(ta1, tc + kWave);
+        for (int tb = tc; tb < tce; tb += G * U) {
+          float v[U][NV];
+          float x[U][B];
 #pragma unroll
-          for (int j = 0; j < NV; ++j) {
-            const int p = 16 * j + o;
-            v[j] = 0.f;
-            if (p < P && ((code >> p) & 1ull))
-              v[j] = m.data[off + popc64(code & ((1ull << p) - 1ull))];
-          }
-          float x[B];
+          for (int u = 0; u < U; ++u) {
+            const int src = (tb - tc + u * G + g) & (kWave - 1);
+            const bool tv = tb + u * G + g < tce;
+            const uint32_t clo = __shfl(mt.clo, src, kWave);
+            const uint32_t chi = __shfl(mt.chi, src, kWave);
+            const int c0 = __shfl(mt.c0, src, kWave);
+            const int off = __shfl(mt.off, src, kWave);
 #pragma unroll
-          for (int c = 0; c < B; ++c) {
-            const int64_t col = (int64_t)c0 + c;
-            x[c] = (tv && fok && col < m.cols) ? X[col * d.ldx + f] : 0.f;
+            for (int j = 0; j < NV; ++j) {
+              const int p = 16 * j + o;
+              const bool has = tv && p < P && code_bit(clo, chi, p);
+              v[u][j] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+            }
+            const int64_t xb = xrow<BUF>(c0, d.ldx) + f;
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+              const bool okc = tv && fok && (BUF || (int64_t)c0 + c < m.cols);
+              x[u][c] = ld.x.f32(xb, okc, (uint32_t)c * ldb);
+            }
           }
-          TileFma<B>::run(acc, v, x);
+#pragma unroll
+          for (int u = 0; u < U; ++u) TileFma<B>::run(acc, v[u], x[u]);
         }
       }
     }
 
     if constexpr (REM) {
-      // Row layout: lanes of row rr walk that row's remainder entries in order; all B rows
-      // advance together so their loads overlap.
-      float racc[FPL];
+      // row lanes: the L lanes of row rr hold FPL columns each and walk that row's entries
+      float racc2[FPL];
 #pragma unroll
-      for (int u = 0; u < FPL; ++u) racc[u] = 0.f;
-      const int64_t row = r0 + rr;
-      int j = 0, j1 = 0;
-      if (row < m.rows) {
-        j = m.rem_ptr[row];
-        j1 = m.rem_ptr[row + 1];
-      }
-      const int kq = q * FPL;
-      for (; __any(j < j1); j += 1) {
-        if (j < j1 && qok) {
-          const float a = m.rem_val[j];
-          const float* xr = X + (int64_t)m.rem_col[j] * d.ldx + kq;
+      for (int i = 0; i < FPL; ++i) racc2[i] = 0.f;
+      const int nmax = wave_max(rs.n);
+      for (int jb = 0; jb < nmax; jb += L) {
+        if (jb > 0) {
+          const bool ok = jb + qq < rs.n;
+          scol = (int)ld.rcol.u32(rs.j0 + jb + qq, ok);
+          sval = ld.rval.f32(rs.j0 + jb + qq, ok);
+        }
+        for (int ib = 0; ib < L && jb + ib < nmax; ib += EB) {
+          float xv[EB][FPL];
+          float a[EB];
 #pragma unroll
-          for (int u = 0; u < FPL; ++u)
-            if (d.k0 + kq + u < d.K) racc[u] = __builtin_fmaf(a, xr[u], racc[u]);
+          for (int i = 0; i < EB; ++i) {
+            const int c = __shfl(scol, gbase + ib + i, kWave);
+            a[i] = __shfl(sval, gbase + ib + i, kWave);
+            const bool ok = jb + ib + i < rs.n;
+            load_row<FPL>(ld.x, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
+          }
+#pragma unroll
+          for (int i = 0; i < EB; ++i)
+#pragma unroll
+            for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(a[i], xv[i][u], racc2[u]);
         }
       }
       // transpose through LDS: (row rr, columns kq..) -> (column f, all rows)
-      if (qok) {
+      if (kq < KW) {
 #pragma unroll
         for (int u = 0; u < FPL; ++u)
-          if (kq + u < KW) trans[wid][rr][kq + u] = racc[u];
+          if (kq + u < KW) ra[rr * KW + kq + u] = racc2[u];
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();
       if (g == 0) {
 #pragma unroll
-        for (int r = 0; r < B; ++r) acc[r] += trans[wid][r][f];
+        for (int r = 0; r < B; ++r) acc[r] += ra[r * KW + f];
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_lds_sync();
     }
 
     // combine the tile groups: every group ends with the full row sums
     if constexpr (G >= 2) {
 #pragma unroll
-      for (int r = 0; r < B; ++r) acc[r] += shfl_xor(acc[r], 32);
+      for (int r = 0; r < B; ++r) acc[r] = xor_sum<32>(acc[r]);
     }
     if constexpr (G >= 4) {
 #pragma unroll
-      for (int r = 0; r < B; ++r) acc[r] += shfl_xor(acc[r], 16);
+      for (int r = 0; r < B; ++r) acc[r] = xor_sum<16>(acc[r]);
     }
 
     epi.begin_stripe(s);
@@ -360,6 +610,13 @@ __global__ void __launch_bounds__(256) k_stripe_feat(PbrView m, RhsView d, Epi e
         if ((r % G) == g && row < m.rows) epi.put(row, colk, acc[r]);
       }
     }
+    // ---- rotate the pipeline
+    ta0 = tb0;
+    ta1 = tb1;
+    tb0 = tc0;
+    tb1 = tc1;
+    mt = mtn;
+    rs = rsn;
   }
   epi.finish(fok ? f : -1, red);
 }
@@ -367,95 +624,123 @@ __global__ void __launch_bounds__(256) k_stripe_feat(PbrView m, RhsView d, Epi e
 // --------------------------------------------------------------------------------------
 // Bit-lane kernel, KW in {1, 2, 4, 8}.
 // --------------------------------------------------------------------------------------
-template <int B, int KW, bool TILES, bool REM, class Epi>
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
 __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi epi) {
   constexpr int P = B * B;
-  constexpr int G = kWave / P;   // tiles per wave
-  constexpr int L = kWave / B;   // lanes per matrix row
+  constexpr int G = kWave / P;      // tiles per wave
+  constexpr int U = B == 8 ? GL_BIT_TILE_U : 2;  // tile groups whose loads go together
+  constexpr int L = kWave / B;      // lanes per matrix row
   __shared__ float red[128];
 
   if (!epi.active()) return;
+  const PbrLd<BUF> ld(m, d);
   const int lane = threadIdx.x & (kWave - 1);
   const int g = lane / P;
   const int p = lane % P;
-  const int rl = p / B;          // row of this lane inside the stripe
-  const int cl = p % B;          // column inside the tile
-  const int q = g * B + cl;      // slot among the L lanes of row rl
-  const uint64_t below = (1ull << p) - 1ull;
-  const float* __restrict__ X = d.x + d.k0;
+  const int rl = p / B;             // row of this lane inside the stripe
+  const int cl = p % B;             // column inside the tile
+  const int q = g * B + cl;         // slot among the L lanes of row rl
   const int kwv = (int)min<int64_t>(KW, d.K - d.k0);  // valid columns of this launch
+  const bool vec = (d.ldx & 3) == 0 && kwv == KW;  // float4 rows, full column chunk
 
   const Sched sc = xcd_sched(m.num_stripes);
-  for (int64_t s = sc.first; s < sc.end; s += sc.stride) {
+  int64_t s = sc.first;
+  int ta0 = 0, ta1 = 0, tb0 = 0, tb1 = 0;
+  if (TILES && s < sc.end) {
+    ta0 = m.stripe_ptr[s];
+    ta1 = m.stripe_ptr[s + 1];
+  }
+  if (TILES && s + sc.stride < sc.end) {
+    tb0 = m.stripe_ptr[s + sc.stride];
+    tb1 = m.stripe_ptr[s + sc.stride + 1];
+  }
+  TileMeta mt{0u, 0u, 0, 0};
+  if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
+  RowSpan rs{0, 0};
+  if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rl);
+
+  for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
+    const int64_t sn = s + sc.stride;
+    const int64_t snn = sn + sc.stride;
+    TileMeta mtn{0u, 0u, 0, 0};
+    int tc0 = 0, tc1 = 0;
+    RowSpan rsn{0, 0};
+    if (TILES) mtn = load_tile_meta(ld, tb0, tb1, lane);
+    if (TILES && snn < sc.end) {
+      tc0 = m.stripe_ptr[snn];
+      tc1 = m.stripe_ptr[snn + 1];
+    }
+    if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rl);
+    // first remainder entry of this lane's row slot
+    int scol = 0;
+    float sval = 0.f;
+    if constexpr (REM) {
+      const bool ok = q < rs.n;
+      scol = (int)ld.rcol.u32(rs.j0 + q, ok);
+      sval = ld.rval.f32(rs.j0 + q, ok);
+    }
+
     float acc[KW];
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = 0.f;
 
     if constexpr (TILES) {
-      const int t0 = m.stripe_ptr[s];
-      const int t1 = m.stripe_ptr[s + 1];
-      for (int tc = t0; tc < t1; tc += kWave) {
-        const int tl = tc + lane;
-        uint64_t mcode = 0;
-        int mc0 = 0, moff = 0;
-        if (tl < t1) {
-          mcode = m.codes[tl];
-          mc0 = m.coords[2 * (int64_t)tl + 1];
-          moff = m.offsets[tl];
-        }
-        const int tce = min(t1, tc + kWave);
-
-        for (int tb = tc; tb < tce; tb += G) {
-          const int src = tb - tc + g;
-          const bool tv = tb + g < tce;
-          const uint32_t clo = __shfl((uint32_t)mcode, src, kWave);
-          const uint32_t chi = __shfl((uint32_t)(mcode >> 32), src, kWave);
-          const int c0 = __shfl(mc0, src, kWave);
-          const int off = __shfl(moff, src, kWave);
-          const uint64_t code = tv ? (((uint64_t)chi << 32) | clo) : 0ull;
-          if ((code >> p) & 1ull) {
-            const int64_t col = (int64_t)c0 + cl;
-            const float v = m.data[off + popc64(code & below)];
-            if (col < m.cols) {
-              const float* xr = X + col * d.ldx;
+      for (int tc = ta0; tc < ta1; tc += kWave) {
+        if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
+        const int tce = min(ta1, tc + kWave);
+        for (int tb = tc; tb < tce; tb += G * U) {
+          float v[U];
+          float xr[U][KW];
 #pragma unroll
-              for (int k = 0; k < KW; ++k)
-                if (k < kwv) acc[k] = __builtin_fmaf(v, xr[k], acc[k]);
-            }
+          for (int u = 0; u < U; ++u) {
+            const int src = (tb - tc + u * G + g) & (kWave - 1);
+            const bool tv = tb + u * G + g < tce;
+            const uint32_t clo = __shfl(mt.clo, src, kWave);
+            const uint32_t chi = __shfl(mt.chi, src, kWave);
+            const int c0 = __shfl(mt.c0, src, kWave);
+            const int off = __shfl(mt.off, src, kWave);
+            const int64_t col = (int64_t)c0 + cl;
+            const bool has = tv && code_bit(clo, chi, p) && (BUF || col < m.cols);
+            v[u] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+            load_row<KW>(ld.x, xrow<BUF>(col, d.ldx), vec, has ? kwv : 0, xr[u]);
           }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(v[u], xr[u][k], acc[k]);
         }
       }
     }
 
     if constexpr (REM) {
-      const int64_t row = r0 + rl;
-      int j = 0, j1 = 0;
-      if (row < m.rows) {
-        j = m.rem_ptr[row] + q;
-        j1 = m.rem_ptr[row + 1];
-      }
-      for (; __any(j < j1); j += L) {
-        if (j < j1) {
-          const float a = m.rem_val[j];
-          const float* xr = X + (int64_t)m.rem_col[j] * d.ldx;
-#pragma unroll
-          for (int k = 0; k < KW; ++k)
-            if (k < kwv) acc[k] = __builtin_fmaf(a, xr[k], acc[k]);
+      // lane (rl, q) takes entries q, q+L, ... of its own row: products go straight into
+      // the lane's accumulators and are summed with the tile products below
+      for (int jb = 0; __any(jb < rs.n); jb += L) {
+        const bool ok = jb + q < rs.n;
+        if (jb > 0) {
+          scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
+          sval = ld.rval.f32(rs.j0 + jb + q, ok);
         }
+        float xv[KW];
+        load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
+#pragma unroll
+        for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(sval, xv[k], acc[k]);
       }
     }
 
-    // sum the L lanes of each row: bits of cl (1 .. B/2) and of g (P .. 32)
+    // sum the lanes of each row: bits of cl (1 .. B/2) and of g (P .. 32)
 #pragma unroll
-    for (int msk = 1; msk < B; msk <<= 1) {
-#pragma unroll
-      for (int k = 0; k < KW; ++k) acc[k] += shfl_xor(acc[k], msk);
-    }
-#pragma unroll
-    for (int msk = P; msk < kWave; msk <<= 1) {
-#pragma unroll
-      for (int k = 0; k < KW; ++k) acc[k] += shfl_xor(acc[k], msk);
+    for (int k = 0; k < KW; ++k) {
+      float t = acc[k];
+      if constexpr (B == 8) t = sum8(t);
+      if constexpr (B == 4) t = xor_sum<2>(xor_sum<1>(t));
+      if constexpr (B == 2) t = xor_sum<1>(t);
+      if constexpr (P <= 4) t = xor_sum<4>(t);
+      if constexpr (P <= 8) t = xor_sum<8>(t);
+      if constexpr (P <= 16) t = xor_sum<16>(t);
+      if constexpr (P <= 32) t = xor_sum<32>(t);
+      acc[k] = t;
     }
 
     epi.begin_stripe(s);
@@ -467,6 +752,12 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
         if (q == k) mine = acc[k];
       epi.put(row, d.k0 + q, mine);
     }
+    ta0 = tb0;
+    ta1 = tb1;
+    tb0 = tc0;
+    tb1 = tc1;
+    mt = mtn;
+    rs = rsn;
   }
   epi.finish(q < kwv ? q : -1, red);
 }

@@ -33,7 +33,8 @@ def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mo
     with torch.cuda.device(x.device):
         rc = L.gl_pbr_spmm_i32_f32(
             pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
-            meta["codes"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+            meta["codes"].numel(), meta["data"].numel(), meta["rem_col_ind"].numel(),
+            meta["codes"].data_ptr(), meta["coords"].data_ptr(),
             meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
             meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
             meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), K, y.data_ptr(),
@@ -93,7 +94,8 @@ class SpmmPlan:
         m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
         self._args = (
             pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
-            meta["codes"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+            meta["codes"].numel(), meta["data"].numel(), meta["rem_col_ind"].numel(),
+            meta["codes"].data_ptr(), meta["coords"].data_ptr(),
             meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
             meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
             meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), x.shape[1], y.data_ptr(),

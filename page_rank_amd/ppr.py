@@ -113,7 +113,7 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
     with torch.cuda.device(device):
         rc = L.gl_ppr_run_i32_f32(
             N, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols, meta["codes"].numel(),
-            meta["codes"].data_ptr(), meta["coords"].data_ptr(), meta["offsets"].data_ptr(),
+            meta["data"].numel(), meta["rem_col_ind"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(), meta["offsets"].data_ptr(),
             meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
             meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
             meta["rem_vals"].data_ptr(), K, ptr.ctypes.data, idx.ctypes.data, w.ctypes.data,

@@ -40,21 +40,21 @@ def test_ctypes_table_matches_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 1
+    assert L.gl_abi_version() == 2
     assert L.gl_last_error() is not None
 
 
 def test_argument_validation_without_device():
     """Argument checks run before any HIP call, so they are testable on the CPU."""
     L = _native.lib()
-    rc = L.gl_pbr_spmm_i32_f32(8, 8, 3, 3, 0, None, None, None, None, None, None, None, None,
-                               None, 1, 1, None, 1, 0, 0, 0, None)
+    rc = L.gl_pbr_spmm_i32_f32(8, 8, 3, 3, 0, 0, 0, None, None, None, None, None, None, None,
+                               None, None, 1, 1, None, 1, 0, 0, 0, None)
     assert rc == _native.GL_ERR_INVALID
     assert "Unsupported block size" in _native.last_error()
     with pytest.raises(ValueError):
         _native.check(rc)
-    rc = L.gl_ppr_run_i32_f32(0, 0, 8, 8, 0, *([None] * 8), 1, None, None, None, None, None,
-                              None, None, None, None, None, None, 0, None, None)
+    rc = L.gl_ppr_run_i32_f32(0, 0, 8, 8, 0, 0, 0, *([None] * 8), 1, None, None, None, None,
+                              None, None, None, None, None, None, None, 0, None, None)
     assert rc == _native.GL_ERR_INVALID
     assert L.gl_ppr_workspace_bytes(100, 8, 4, 3) > 100 * 4 * 4
     assert L.gl_ppr_workspace_bytes(-1, 8, 4, 3) == -1
