@@ -186,6 +186,50 @@ int gl_pbr_permute_tiles(int64_t num_tiles, const int32_t* perm, const uint64_t*
                          int32_t* offsets_out, void* data_out, void* workspace,
                          int64_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference; north-star learned part).
+ * MLP 12 -> 64 -> 64 -> 1, ReLU, ReLU, softplus; parameters packed as
+ * W1[64][12] b1[64] W2[64][64] b2[64] w3[64] b3 (gl_mlp_num_params() floats).
+ * Features of (node i, seed set k): degree buckets round(8 log2(1+deg))/8 (in, out), one-hot
+ * hop 0..4/more, log1p(N q_t)/8 for the first three power-iteration terms q_1..q_3, and 1.
+ * ---------------------------------------------------------------------------------- */
+int32_t gl_mlp_num_params(void);
+int32_t gl_mlp_num_features(void);
+int32_t gl_ws_num_buckets(void);
+/* out[m] = softplus(MLP(feat[m])) for feat [M][12]. */
+int gl_mlp_forward(const float* params, const float* feat, int64_t M, float* out, void* stream);
+/* Floats of the per-block gradient buffer the training calls need. */
+int64_t gl_mlp_train_partials(void);
+/* One Adam step on the weighted L1 loss sum_m weight[m] |s_m - target[m]| over M samples,
+ * gradient multiplied by grad_scale; *loss += the loss. */
+int gl_mlp_train_step(float* params, const float* feat, const float* target,
+                      const float* weight, int64_t M, float grad_scale, float lr, float* adam_m,
+                      float* adam_v, int32_t step, float* partial, float* loss, void* stream);
+/* Gradient of the same loss into grad[gl_mlp_num_params()] (no update). */
+int gl_mlp_grad(const float* params, const float* feat, const float* target, const float* weight,
+                int64_t M, float* partial, float* grad, float* loss, void* stream);
+/* In- and out-degree of every node from the PBR matrix (row i = in-edges of i). */
+int gl_ws_degrees(int64_t n, int32_t block_rows, int64_t num_tiles, const uint64_t* codes,
+                  const int32_t* coords, const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                  int32_t* indeg, int32_t* outdeg, void* stream);
+/* Features [M][12] of the listed pairs (pair = i*K + k); v5 = 5 device pointers to
+ * A^j P, j = 0..4, each [N][K]. */
+int gl_ws_pair_features(int64_t N, int64_t K, float alpha, const int32_t* indeg,
+                        const int32_t* outdeg, const float* const* v5, const int64_t* pairs,
+                        int64_t M, float* feat, void* stream);
+/* Features of the (in-bucket, out-bucket, hop class 3/4/more) table, gl_ws_table_size() rows. */
+int64_t gl_ws_table_size(void);
+int gl_ws_table_features(float* feat, void* stream);
+/* X0 from the table for pairs beyond hop 2; pairs within hop 2 appended to near (count in
+ * *near_count on the device, list truncated at near_cap). */
+int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
+                   const int32_t* outdeg, const float* const* v5, const float* table, float* x0,
+                   int64_t* near, int64_t near_cap, uint64_t* near_count, void* stream);
+/* x0[pairs[m]] = val[m]. */
+int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, void* stream);
+/* Scale every column of x [N][K] to sum 1 (sums: K floats of scratch). */
+int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

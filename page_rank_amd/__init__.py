@@ -13,6 +13,7 @@ from .ops import (SpmmPlan, pbr_batched_matmul_cpu, pbr_batched_matmul_cuda, pbr
 from .ppr import (PPRResult, normalize_transition_matrix, run_personalized_pagerank,
                   run_ppr)
 from .builder import csr_to_pbr, pbr_to_csr
+from . import graphs, warmstart
 
 __all__ = [
     "csr_to_pbr",
@@ -33,4 +34,6 @@ __all__ = [
     "PBRMatrixInt64Float",
     "PBRMatrixInt32Double",
     "PBRMatrixInt64Double",
+    "graphs",
+    "warmstart",
 ]

@@ -79,6 +79,23 @@ _PROTOS = {
     "gl_pbr_permute_workspace_bytes": (c_i64, [c_i64]),
     "gl_pbr_permute_tiles": (ctypes.c_int, [c_i64, c_p, c_p, c_p, c_p, c_p, c_i32, c_p, c_p, c_p,
                                             c_p, c_p, c_i64, c_p]),
+    "gl_mlp_num_params": (c_i32, []),
+    "gl_mlp_num_features": (c_i32, []),
+    "gl_ws_num_buckets": (c_i32, []),
+    "gl_mlp_forward": (ctypes.c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "gl_mlp_train_partials": (c_i64, []),
+    "gl_mlp_train_step": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_i64, c_f32, c_f32, c_p, c_p, c_i32,
+                                         c_p, c_p, c_p]),
+    "gl_mlp_grad": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_i64, c_p, c_p, c_p, c_p]),
+    "gl_ws_degrees": (ctypes.c_int, [c_i64, c_i32, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "gl_ws_pair_features": (ctypes.c_int, [c_i64, c_i64, c_f32, c_p, c_p, c_p, c_p, c_i64, c_p,
+                                           c_p]),
+    "gl_ws_table_size": (c_i64, []),
+    "gl_ws_table_features": (ctypes.c_int, [c_p, c_p]),
+    "gl_ws_assemble": (ctypes.c_int, [c_i64, c_i64, c_f32, c_p, c_p, c_p, c_p, c_p, c_p, c_i64,
+                                      c_p, c_p]),
+    "gl_ws_scatter": (ctypes.c_int, [c_p, c_p, c_i64, c_p, c_p]),
+    "gl_ws_normalize": (ctypes.c_int, [c_p, c_i64, c_i64, c_p, c_p]),
 }
 
 _lib = None

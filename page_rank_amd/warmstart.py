@@ -1,0 +1,314 @@
+"""Learned warm start for batched PPR.
+
+A 12 -> 64 -> 64 -> 1 MLP (ReLU, ReLU, softplus) maps per-(node, seed set) features to an
+unnormalized score; each seed set's column is normalized to sum 1 and used as the start
+vector of the power iteration (run_ppr(..., x0=...)). The plain seed-start iteration is the
+judge: the warm start must reach the same residual, and what it buys is iterations and wall
+time. Every kernel (degrees, SpMM for the feature terms, features, forward, backward, Adam,
+assembly) is in libgraphlink_hip.so.
+
+Features: degree buckets (in, out), one-hot hop distance to the seed set (0..4, more), the
+first three power-iteration terms q_1..q_3 as log1p(N q)/8, and a constant. Pairs beyond
+two hops depend only on (degree buckets, hop class), so they read a table evaluated once;
+pairs within two hops are evaluated one by one.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native
+from .ops import spmm_into
+from .registry import get_pbr_gpu_meta
+
+
+def _lib():
+    return _native.lib()
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def init_params(seed=0, device="cuda"):
+    """He-initialized parameters (bias of the output set so softplus starts near 1)."""
+    L = _lib()
+    n_in, hid = L.gl_mlp_num_features(), 64
+    g = np.random.default_rng(seed)
+    W1 = g.normal(0, np.sqrt(2 / n_in), (hid, n_in))
+    W2 = g.normal(0, np.sqrt(2 / hid), (hid, hid))
+    w3 = g.normal(0, np.sqrt(1 / hid), hid)
+    p = np.concatenate([W1.ravel(), np.zeros(hid), W2.ravel(), np.zeros(hid), w3, [0.5]])
+    assert p.size == L.gl_mlp_num_params()
+    return torch.as_tensor(p.astype(np.float32), device=device)
+
+
+@dataclass
+class WsInputs:
+    """Device inputs of the feature kernels for one graph and one batch of seed sets."""
+    N: int
+    K: int
+    alpha: float
+    indeg: torch.Tensor
+    outdeg: torch.Tensor
+    v: list = field(default_factory=list)  # A^j P, j = 0..4, each [N][K]
+    ms: float = 0.0
+
+    def vptrs(self):
+        return _ptrs(self.v)
+
+
+def degrees(pbr):
+    meta = get_pbr_gpu_meta(pbr)
+    if meta is None:
+        raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    if pbr.block_rows != pbr.block_cols or pbr.rows != pbr.cols:
+        raise ValueError("degrees need a square matrix with square blocks")
+    dev = meta["codes"].device
+    n = pbr.rows
+    indeg = torch.empty(n, dtype=torch.int32, device=dev)
+    outdeg = torch.empty(n, dtype=torch.int32, device=dev)
+    _native.check(_lib().gl_ws_degrees(n, pbr.block_rows, meta["codes"].numel(),
+                                       meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+                                       meta["rem_indptr"].data_ptr(),
+                                       meta["rem_col_ind"].data_ptr(), indeg.data_ptr(),
+                                       outdeg.data_ptr(), _native.stream_handle(dev)),
+                  "gl_ws_degrees")
+    return indeg, outdeg
+
+
+def seed_matrix(n, seed_sets, device):
+    P = torch.zeros((n, len(seed_sets)), dtype=torch.float32, device=device)
+    for k, s in enumerate(seed_sets):
+        s = torch.as_tensor(np.asarray(s, dtype=np.int64), device=device)
+        P[:, k].index_add_(0, s, torch.full((s.numel(),), 1.0 / s.numel(), device=device))
+    return P
+
+
+def inputs(pbr, seed_sets, alpha=0.85, degs=None) -> WsInputs:
+    meta = get_pbr_gpu_meta(pbr)
+    if meta is None:
+        raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    dev = meta["codes"].device
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    indeg, outdeg = degrees(pbr) if degs is None else degs
+    v = [seed_matrix(pbr.rows, seed_sets, dev)]
+    for _ in range(4):
+        y = torch.empty_like(v[0])
+        spmm_into(pbr, v[-1], y)
+        v.append(y)
+    torch.cuda.synchronize(dev)
+    return WsInputs(pbr.rows, len(seed_sets), alpha, indeg, outdeg, v,
+                    (time.perf_counter() - t0) * 1e3)
+
+
+def forward(params, feat):
+    out = torch.empty(feat.shape[0], dtype=torch.float32, device=feat.device)
+    _native.check(_lib().gl_mlp_forward(params.data_ptr(), feat.data_ptr(), feat.shape[0],
+                                        out.data_ptr(), _native.stream_handle(feat.device)),
+                  "gl_mlp_forward")
+    return out
+
+
+def pair_features(wi: WsInputs, pairs):
+    dev = wi.v[0].device
+    feat = torch.empty((pairs.numel(), 12), dtype=torch.float32, device=dev)
+    _native.check(_lib().gl_ws_pair_features(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
+                                             wi.outdeg.data_ptr(), wi.vptrs(),
+                                             pairs.data_ptr(), pairs.numel(), feat.data_ptr(),
+                                             _native.stream_handle(dev)),
+                  "gl_ws_pair_features")
+    return feat
+
+
+def predict(params, wi: WsInputs, near_cap=None):
+    """Normalized warm-start matrix X0 [N][K] and the number of per-pair evaluations."""
+    L = _lib()
+    dev = wi.v[0].device
+    st = _native.stream_handle(dev)
+    tsz = L.gl_ws_table_size()
+    tfeat = torch.empty((tsz, 12), dtype=torch.float32, device=dev)
+    _native.check(L.gl_ws_table_features(tfeat.data_ptr(), st), "gl_ws_table_features")
+    table = forward(params, tfeat)
+    x0 = torch.empty((wi.N, wi.K), dtype=torch.float32, device=dev)
+    cap = near_cap if near_cap is not None else max(1 << 16, wi.N * wi.K // 4)
+    near = torch.empty(cap, dtype=torch.int64, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
+                                   wi.outdeg.data_ptr(), wi.vptrs(), table.data_ptr(),
+                                   x0.data_ptr(), near.data_ptr(), cap, cnt.data_ptr(), st),
+                  "gl_ws_assemble")
+    n_near = int(cnt.item())
+    if n_near > cap:
+        return predict(params, wi, near_cap=n_near)
+    pairs = near[:n_near]
+    if n_near:
+        vals = forward(params, pair_features(wi, pairs))
+        _native.check(L.gl_ws_scatter(pairs.data_ptr(), vals.data_ptr(), n_near, x0.data_ptr(), st),
+                      "gl_ws_scatter")
+    sums = torch.empty(wi.K, dtype=torch.float32, device=dev)
+    _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
+                  "gl_ws_normalize")
+    return x0, n_near
+
+
+def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0):
+    """Training set of one graph: every pair within two hops plus `far_samples` uniformly
+    drawn pairs (weight N*K / far_samples, which makes the weighted loss an unbiased
+    estimate of the full-column L1). Targets are N * x*."""
+    L = _lib()
+    dev = wi.v[0].device
+    st = _native.stream_handle(dev)
+    cap = max(1 << 16, wi.N * wi.K // 4)
+    near = torch.empty(cap, dtype=torch.int64, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    dummy_table = torch.zeros(L.gl_ws_table_size(), dtype=torch.float32, device=dev)
+    x0 = torch.empty((wi.N, wi.K), dtype=torch.float32, device=dev)
+    _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
+                                   wi.outdeg.data_ptr(), wi.vptrs(), dummy_table.data_ptr(),
+                                   x0.data_ptr(), near.data_ptr(), cap, cnt.data_ptr(), st),
+                  "gl_ws_assemble")
+    n_near = min(int(cnt.item()), cap)
+    del x0
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    far = torch.randint(0, wi.N * wi.K, (far_samples,), generator=g).to(dev)
+    pairs = torch.cat([near[:n_near], far])
+    feat = pair_features(wi, pairs)
+    # drop far draws that landed within two hops (they are already in the near list)
+    hop_far = feat[n_near:, 2:5].sum(1) == 0
+    keep = torch.cat([torch.ones(n_near, dtype=torch.bool, device=dev), hop_far])
+    pairs, feat = pairs[keep], feat[keep]
+    n_far = int(hop_far.sum().item())
+    w = torch.ones(pairs.numel(), dtype=torch.float32, device=dev)
+    w[n_near:] = float(wi.N * wi.K) / far_samples
+    target = x_star.reshape(-1)[pairs] * wi.N
+    return feat, target, w, n_near, n_far
+
+
+@dataclass
+class TrainLog:
+    losses: list
+    seconds: float
+
+
+def train(params, feat, target, weight, *, epochs=8, batch=1 << 18, lr=2e-3, seed=0):
+    """Adam on the weighted L1 loss with minibatches; all work in HIP kernels."""
+    L = _lib()
+    dev = feat.device
+    st = _native.stream_handle(dev)
+    m1 = torch.zeros_like(params)
+    m2 = torch.zeros_like(params)
+    partial = torch.empty(L.gl_mlp_train_partials(), dtype=torch.float32, device=dev)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    M = feat.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    step = 0
+    losses = []
+    t0 = time.perf_counter()
+    total_w = float(weight.sum().item())
+    for ep in range(epochs):
+        perm = torch.randperm(M, generator=g).to(dev)
+        loss.zero_()
+        for b in range(0, M, batch):
+            idx = perm[b:b + batch]
+            f, t, w = feat[idx].contiguous(), target[idx].contiguous(), weight[idx].contiguous()
+            step += 1
+            scale = 1.0 / max(float(idx.numel()), 1.0) * (M / total_w)
+            _native.check(L.gl_mlp_train_step(params.data_ptr(), f.data_ptr(), t.data_ptr(),
+                                              w.data_ptr(), idx.numel(), scale, lr,
+                                              m1.data_ptr(), m2.data_ptr(), step,
+                                              partial.data_ptr(), loss.data_ptr(), st),
+                          "gl_mlp_train_step")
+        losses.append(float(loss.item()) / total_w)
+    torch.cuda.synchronize(dev)
+    return TrainLog(losses, time.perf_counter() - t0)
+
+
+def grad(params, feat, target, weight):
+    L = _lib()
+    dev = feat.device
+    partial = torch.empty(L.gl_mlp_train_partials(), dtype=torch.float32, device=dev)
+    g = torch.empty(L.gl_mlp_num_params(), dtype=torch.float32, device=dev)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    _native.check(L.gl_mlp_grad(params.data_ptr(), feat.data_ptr(), target.data_ptr(),
+                                weight.data_ptr(), feat.shape[0], partial.data_ptr(),
+                                g.data_ptr(), loss.data_ptr(), _native.stream_handle(dev)),
+                  "gl_mlp_grad")
+    return g, float(loss.item())
+
+
+def save(params, path):
+    np.savez(path, params=params.detach().cpu().numpy())
+
+
+def load(path, device="cuda"):
+    with np.load(path) as z:
+        return torch.as_tensor(z["params"], device=device)
+
+
+def fit(samples, params=None, *, alpha=0.85, far_samples=1 << 18, epochs=8, batch=1 << 16,
+        lr=2e-3, tol=1e-7, max_iter=400, seed=0):
+    """Train on a list of (pbr, seed_sets, dangling_mask, dangling_mode) graphs: the
+    exact solution of each is computed by run_ppr, then pairs are drawn by
+    training_pairs and the network is fitted with train. Returns (params, TrainLog)."""
+    from .ppr import run_ppr
+    feats, targets, weights = [], [], []
+    dev = None
+    for j, (pbr, sets, dang, dmode) in enumerate(samples):
+        wi = inputs(pbr, sets, alpha)
+        dev = wi.v[0].device
+        ref = run_ppr(pbr, sets, alpha=alpha, tol=tol, max_iter=max_iter, dangling=dang,
+                      dangling_mode=dmode, return_history=False)
+        f, t, w, _, _ = training_pairs(wi, ref.X, far_samples, seed=seed + j)
+        feats.append(f), targets.append(t), weights.append(w)
+    if params is None:
+        params = init_params(seed, dev)
+    log = train(params, torch.cat(feats), torch.cat(targets), torch.cat(weights),
+                epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return params, log
+
+
+@dataclass
+class WarmStartEval:
+    """Plain (seed start) against warm start at the same tolerance."""
+    plain_iters: np.ndarray
+    warm_iters: np.ndarray
+    plain_ms: float
+    warm_ms: float          # features + network + power iteration
+    predict_ms: float       # features + network alone
+    start_l1_plain: np.ndarray   # ||x0 - x*||_1 per column
+    start_l1_warm: np.ndarray
+    max_l1_diff: float      # between the two converged results
+
+
+def evaluate(params, pbr, seed_sets, dangling=None, dangling_mode="none", *, alpha=0.85,
+             tol=1e-6, max_iter=400, x_star=None):
+    from .ppr import run_ppr
+    dev = get_pbr_gpu_meta(pbr)["codes"].device
+    kw = dict(alpha=alpha, tol=tol, max_iter=max_iter, dangling=dangling,
+              dangling_mode=dangling_mode, return_history=False)
+    run_ppr(pbr, seed_sets, **kw)                       # warm up graph capture / caches
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    plain = run_ppr(pbr, seed_sets, **kw)
+    torch.cuda.synchronize(dev)
+    t1 = time.perf_counter()
+    wi = inputs(pbr, seed_sets, alpha)
+    x0, _ = predict(params, wi)
+    torch.cuda.synchronize(dev)
+    t2 = time.perf_counter()
+    warm = run_ppr(pbr, seed_sets, x0=x0, **kw)
+    torch.cuda.synchronize(dev)
+    t3 = time.perf_counter()
+    if x_star is None:
+        x_star = run_ppr(pbr, seed_sets, **{**kw, "tol": min(tol, 1e-7)}).X
+    P = seed_matrix(pbr.rows, seed_sets, dev)
+    return WarmStartEval(
+        plain.iterations, warm.iterations, (t1 - t0) * 1e3, (t3 - t1) * 1e3, (t2 - t1) * 1e3,
+        (P - x_star).abs().sum(0).cpu().numpy(), (x0 - x_star).abs().sum(0).cpu().numpy(),
+        float((plain.X - warm.X).abs().sum(0).max().item()))
