@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only-headline", action="store_true",
                     help="time only the headline width (for profiler runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB (a kernel of "
+                         "known traffic that scales the profiler's FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -123,6 +126,14 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"calibration copies of {args.calibrate_copy_mib} MiB done")
     log("generating the 1M x 1M SpMM matrix ...")
     A = graphs.spmm_matrix(seed=0)
     log(f"  nnz={A.nnz}")

@@ -72,7 +72,8 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
                         float* Y, int64_t ldy, int32_t accumulate, int32_t mode, int32_t kw,
                         void* stream);
 
-/* Block-tile part alone (north-star subsystem b). Rows with no tile get 0 (or keep Y). */
+/* Block-tile part alone. Rows with no tile get 0 (or keep Y). Replaces launch_pbr_spmm
+ * (Backend/graph_link/kernels/pbr_kernel.cu:336-396). */
 int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
                               int32_t block_cols, int64_t num_tiles, int64_t num_vals,
                               const uint64_t* codes,
@@ -81,7 +82,7 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
                               int64_t K, float* Y, int64_t ldy, int32_t accumulate, int32_t kw,
                               void* stream);
 
-/* CSR SpMM alone (north-star subsystem c; the PBR remainder, or any CSR matrix).
+/* CSR SpMM alone (the PBR remainder, or any CSR matrix).
  * Replaces launch_csr_spmm (Backend/graph_link/kernels/pbr_kernel.cu:404-478). */
 int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* indptr,
                         const int32_t* col_ind, const float* vals, const float* X, int64_t ldx,
@@ -89,7 +90,7 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* 
                         void* stream);
 
 /* ------------------------------------------------------------------------------------
- * Personalized PageRank power iteration (north-star subsystem d + the persistent loop).
+ * Personalized PageRank power iteration (fused step + the persistent loop).
  * Replaces run_personalized_pagerank (Backend/graph_link/graph_link/__init__.py:210-271),
  * init_ppr (Backend/graph_link/kernels/ppr_kernel.cu:199-216), ppr_update_normalized
  * (:130-187) and the dangling->seed step missing_mass + ppr_update (:13-117).
@@ -144,7 +145,7 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
 
 
 /* ------------------------------------------------------------------------------------
- * GPU PBR builder (north-star subsystem a). Replaces csr_to_pbr
+ * GPU PBR builder. Replaces csr_to_pbr
  * (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477, wrapper
  * Backend/graph_link/graph_link/__init__.py:137-156). Input: canonical CSR on the device
  * (int32 indptr/indices, rows sorted by column, no duplicates); values are copied as opaque
@@ -187,7 +188,7 @@ int gl_pbr_permute_tiles(int64_t num_tiles, const int32_t* perm, const uint64_t*
                          int64_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------------------
- * Learned warm start (no counterpart in the reference; north-star learned part).
+ * Learned warm start (no counterpart in the reference).
  * MLP 12 -> 64 -> 64 -> 1, ReLU, ReLU, softplus; parameters packed as
  * W1[64][12] b1[64] W2[64][64] b2[64] w3[64] b3 (gl_mlp_num_params() floats).
  * Features of (node i, seed set k): degree buckets round(8 log2(1+deg))/8 (in, out), one-hot

@@ -8,8 +8,8 @@ All compute runs in hand-written HIP kernels of ``lib/libgraphlink_hip.so`` thro
 from .pbr import (DataOrder, PBRMatrix, PBRMatrixInt32Double, PBRMatrixInt32Float,
                   PBRMatrixInt64Double, PBRMatrixInt64Float, from_arrays)
 from .registry import clear_pbr_gpu_meta, get_pbr_gpu_meta, set_pbr_gpu_meta
-from .ops import (SpmmPlan, pbr_batched_matmul_cpu, pbr_batched_matmul_cuda, pbr_matmul,
-                  spmm_into)
+from .ops import (SpmmPlan, block_spmm_into, pbr_batched_matmul_cpu, pbr_batched_matmul_cuda,
+                  pbr_batched_matmul_triton_gpu, pbr_matmul, spmm_into)
 from .ppr import (PPRResult, normalize_transition_matrix, run_personalized_pagerank,
                   run_ppr)
 from .builder import csr_to_pbr, pbr_to_csr
@@ -21,6 +21,7 @@ __all__ = [
     "pbr_matmul",
     "pbr_batched_matmul_cpu",
     "pbr_batched_matmul_cuda",
+    "pbr_batched_matmul_triton_gpu",
     "get_pbr_gpu_meta",
     "clear_pbr_gpu_meta",
     "normalize_transition_matrix",

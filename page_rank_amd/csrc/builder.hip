@@ -1,4 +1,4 @@
-// GPU PBR builder (north-star subsystem a): canonical CSR -> PBR tiles + CSR remainder,
+// GPU PBR builder: canonical CSR -> PBR tiles + CSR remainder,
 // bit-exact with the reference builder (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477).
 //
 // No sort: a group of GL lanes owns one stripe (lane r <-> matrix row r0 + r). Each row's

@@ -244,7 +244,7 @@ struct PprCtl {
   int32_t pad[2];
 };
 
-// Fused power-iteration step (north-star subsystem d):
+// Fused power-iteration step:
 //   X_{t+1}[i,k] = alpha * (A X_t)[i,k] + cu[k] + tele[k] * p_k[i]
 // with cu[k] = alpha * dangling_k / N (uniform dangling) and tele[k] = (1 - alpha)
 // (+ alpha * dangling_k in personalization mode), frozen columns copied unchanged, the L1

@@ -71,6 +71,45 @@ def pbr_matmul(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0):
     return y.squeeze(0) if x.dim() == 2 else y
 
 
+def block_spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=True, kw=0):
+    """y (+)= A_tiles @ x: the block tiles only, the CSR remainder left out."""
+    meta = _meta_or_raise(pbr_mat)
+    if x.dtype != torch.float32 or y.dtype != torch.float32:
+        raise TypeError("block SpMM needs float32 x and y")
+    if x.dim() != 2 or x.shape[0] != pbr_mat.cols or y.shape != (pbr_mat.rows, x.shape[1]):
+        raise ValueError(f"shape mismatch: A is {pbr_mat.rows}x{pbr_mat.cols}, x {tuple(x.shape)}"
+                         f", y {tuple(y.shape)}")
+    if x.stride(1) != 1 or y.stride(1) != 1:
+        raise ValueError("x and y must have unit stride along K")
+    L = _native.lib()
+    with torch.cuda.device(x.device):
+        rc = L.gl_pbr_block_spmm_i32_f32(
+            pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
+            meta["codes"].numel(), meta["data"].numel(), meta["codes"].data_ptr(),
+            meta["coords"].data_ptr(), meta["offsets"].data_ptr(), meta["data"].data_ptr(),
+            meta["stripe_ptr"].data_ptr(), x.data_ptr(), x.stride(0), x.shape[1], y.data_ptr(),
+            y.stride(0), 1 if accumulate else 0, kw, _native.stream_handle(x.device))
+    _native.check(rc, "gl_pbr_block_spmm_i32_f32")
+    return y
+
+
+def pbr_batched_matmul_triton_gpu(pbr_mat, x: torch.Tensor, y: torch.Tensor, batch_size: int,
+                                  width: int):
+    """Name and semantics of the reference's block-only product
+    (Backend/graph_link/graph_link/pbr_matrix_triton.py:77-110): y += A_tiles @ x over the
+    block tiles (the CSR remainder is not included), x and y [rows, width] or
+    [batch, rows, width]. Runs the HIP block kernel; there is no Triton in this build."""
+    if x.dim() == 2:
+        pairs = [(x, y)]
+    else:
+        pairs = [(x[b], y[b]) for b in range(batch_size)]
+    for xb, yb in pairs:
+        if xb.shape[-1] != width:
+            raise ValueError(f"width {width} does not match x {tuple(xb.shape)}")
+        block_spmm_into(pbr_mat, xb, yb, accumulate=True)
+    return y
+
+
 def pbr_batched_matmul_cpu(pbr_mat, x):
     raise NotImplementedError("pbr_batched_matmul_cpu: this build has no CPU SpMM; use "
                               "pbr_matmul with a GPU tensor")

@@ -2,9 +2,9 @@
 
 A 12 -> 64 -> 64 -> 1 MLP (ReLU, ReLU, softplus) maps per-(node, seed set) features to an
 unnormalized score; each seed set's column is normalized to sum 1 and used as the start
-vector of the power iteration (run_ppr(..., x0=...)). The plain seed-start iteration is the
-judge: the warm start must reach the same residual, and what it buys is iterations and wall
-time. Every kernel (degrees, SpMM for the feature terms, features, forward, backward, Adam,
+vector of the power iteration (run_ppr(..., x0=...)). The plain seed-start iteration is both
+the fallback and the yardstick: the warm start must reach the same residual, and what it buys
+is iterations and wall time. Every kernel (degrees, SpMM for the feature terms, features, forward, backward, Adam,
 assembly) is in libgraphlink_hip.so.
 
 Features: degree buckets (in, out), one-hot hop distance to the seed set (0..4, more), the

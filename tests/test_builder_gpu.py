@@ -1,4 +1,4 @@
-"""GPU PBR builder (north-star subsystem a): bit-exact against the CPU restatement of the
+"""GPU PBR builder: bit-exact against the CPU restatement of the
 reference builder (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:340-477), in the
 reference's emission order and in canonical order."""
 import numpy as np

@@ -1,4 +1,4 @@
-"""GPU Personalized PageRank (north-star subsystem d and the persistent loop).
+"""GPU Personalized PageRank (fused step and the persistent loop).
 
 test_ppr_accuracy mirrors Tests/test_ppr_accuracy.py:58-96 (networkx oracle, atol 1e-3,
 mass 1 +- 1e-4); the remaining tests pin the batched path against the FP64 oracle

@@ -1,10 +1,11 @@
-"""GPU SpMM parity (north-star subsystems b and c) against scipy and the CPU oracle.
+"""GPU SpMM parity (block tiles and CSR remainder) against scipy and the CPU oracle.
 
 Grid of Tests/test_spmm.py:37-88 (N x K x density x block x min_nnz, plus non-power-of-two K),
 tolerance atol=1e-4 as the reference (Tests/test_spmm.py:56), plus a tighter relative check
 against the oracle's float64-accumulated product.
 """
 import numpy as np
+import scipy.sparse as sp
 import pytest
 import torch
 
@@ -134,3 +135,42 @@ def test_errors(device):
     odd = graph_link.csr_to_pbr(A, 3, 3, 1)
     with pytest.raises(ValueError):
         graph_link.pbr_matmul(odd, torch.zeros((64, 2), device=device))
+
+
+def test_to_semantics(device):
+    """Backend/graph_link/graph_link/__init__.py:17-56: same device returns self; another
+    device returns a new object registered by id(); the registry entry dies with it."""
+    A = random_csr(100, 0.05, seed=2)
+    built = graph_link.csr_to_pbr(A, 4, 4, 2)          # built on the GPU
+    assert built.to(device) is built
+    host = built.to("cpu")
+    assert host is not built and graph_link.get_pbr_gpu_meta(host) is None
+    assert host.to("cpu") is host
+    gpu = host.to(device)
+    assert gpu is not host and graph_link.get_pbr_gpu_meta(gpu) is not None
+    assert graph_link.get_pbr_gpu_meta(host) is None
+    assert gpu.to(device) is gpu
+    assert gpu.to("cuda") is gpu
+    meta = graph_link.get_pbr_gpu_meta(gpu)
+    assert meta["codes"].dtype == torch.int64 and meta["coords"].dtype == torch.int32
+    assert np.array_equal(meta["codes"].cpu().numpy().view(np.uint64), host.block_codes)
+    graph_link.clear_pbr_gpu_meta(gpu)
+    assert graph_link.get_pbr_gpu_meta(gpu) is None
+
+
+def test_block_only_product_named_like_the_reference(device):
+    """pbr_batched_matmul_triton_gpu (pbr_matrix_triton.py:77-110): y += tiles @ x."""
+    A = blocky_csr(256, 8, 3, 2, 40, scatter=500, seed=5)
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
+    o = opbr.csr_matrix_to_pbr(A, 8, 8, 4, order="canonical")
+    tiles_only = A - sp.csr_matrix((o.remainder_data, o.remainder_col_ind, o.remainder_indptr),
+                                   shape=A.shape)
+    X = random_x(256, 12)
+    y = torch.ones((256, 12), device=device)
+    graph_link.pbr_batched_matmul_triton_gpu(pbr, torch.from_numpy(X).to(device), y, 1, 12)
+    assert np.allclose(y.cpu().numpy(), tiles_only.dot(X) + 1.0, atol=1e-4)
+    X3 = np.stack([random_x(256, 12, seed=s) for s in range(2)])
+    y3 = torch.zeros((2, 256, 12), device=device)
+    graph_link.pbr_batched_matmul_triton_gpu(pbr, torch.from_numpy(X3).to(device), y3, 2, 12)
+    for b in range(2):
+        assert np.allclose(y3[b].cpu().numpy(), tiles_only.dot(X3[b]), atol=1e-4)
