@@ -44,6 +44,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--only-headline", action="store_true",
                     help="time only the headline width (for profiler runs)")
+    ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
+    ap.add_argument("--ppr-only", action="store_true", help="run only the batched-PPR leg")
+    ap.add_argument("--ppr-nodes", type=int, default=4_000_000)
+    ap.add_argument("--ppr-edges", type=int, default=64_000_000)
+    ap.add_argument("--ppr-sets", type=int, default=64)
+    ap.add_argument("--ppr-check-cols", type=int, default=2,
+                    help="columns checked against the FP64 CPU power iteration (and timed "
+                         "as the CPU baseline of the PPR leg)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a kernel of "
                          "known traffic that scales the profiler's FETCH_SIZE / WRITE_SIZE)")
@@ -112,6 +120,86 @@ def cpu_baseline(A, K, seconds):
                       f"in {t:.1f} s, oracle/pbr_oracle.cpp spmm (float accumulation)"}
 
 
+def ppr_matrix_bytes(T, V, R, N):
+    return 20 * T + 4 * V + 4 * (N + 1) + 8 * R
+
+
+def ppr_leg(args, dev):
+    """Batched multi-seed PPR on the block-clustered power-law graph (4M nodes, 64M edges,
+    4,000 communities, 80 % intra, out-degree Pareto(2.1, min 4)), 64 seed sets of U{1..100}
+    nodes, alpha 0.85, per-column L1 tol 1e-6, hipGraph-captured loop. Errors against the
+    FP64 CPU power iteration on sampled columns, which is also the CPU baseline."""
+    import numpy as np
+    import torch
+    import page_rank_amd as gl
+    from page_rank_amd import graphs
+    n, m, K = args.ppr_nodes, args.ppr_edges, args.ppr_sets
+    log(f"generating the {n} node / {m} edge clustered graph ...")
+    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000), seed=3)
+    A, dang = graphs.transition_matrix(src, dst, n)
+    del src, dst
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pbr = gl.csr_to_pbr(A, 8, 8, 4, order="canonical")
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    T, V, R = pbr.accounted_blocks(), pbr.compressed_nnz(), pbr.remainder_nnz()
+    log(f"  nnz={A.nnz} T={T} V={V} R={R} dangling={int(dang.sum())}, built in "
+        f"{t_build*1e3:.0f} ms (incl. upload)")
+    sets = graphs.seed_sets(n, K, 1, 100, seed=4)
+    dmode = "uniform" if dang.any() else "none"
+    kw = dict(alpha=0.85, tol=1e-6, max_iter=300, dangling=dang if dang.any() else None,
+              dangling_mode=dmode, retire="per_column", return_history=False)
+    gl.run_ppr(pbr, sets, **kw)
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = gl.run_ppr(pbr, sets, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    col_iters = int(res.iterations.sum())
+    nbytes = res.total_iterations * ppr_matrix_bytes(T, V, R, n) + 8 * n * col_iters
+    gbs = nbytes / t / 1e9
+    log(f"  plain: {K / t:.1f} PPR/s, {t*1e3:.1f} ms, iterations max {res.total_iterations} "
+        f"mean {res.iterations.mean():.1f}, {gbs:.0f} GB/s algorithmic")
+    out = {"workload": f"ppr-{n // 1_000_000}M-{m // 1_000_000}M-clustered-{K}sets",
+           "nodes": n, "edges_nnz": int(A.nnz), "tiles": T, "tile_nnz": V, "remainder_nnz": R,
+           "seed_sets": K, "seed_set_size": "U{1..100}", "alpha": 0.85, "tol": 1e-6,
+           "dangling_mode": dmode, "ppr_per_s": round(K / t, 3), "ms": round(t * 1e3, 3),
+           "iterations_max": res.total_iterations,
+           "iterations_mean": round(float(res.iterations.mean()), 2),
+           "converged": int(res.converged.sum()),
+           "algorithmic_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "build_ms": round(t_build * 1e3, 1)}
+    if args.ppr_check_cols and not args.no_cpu:
+        from oracle import ppr as oppr
+        cols = np.random.default_rng(5).choice(K, min(K, args.ppr_check_cols), replace=False)
+        P = oppr.personalization_matrix(n, [sets[c] for c in cols])
+        t0 = time.perf_counter()
+        X64, it64, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-6, max_iter=300,
+                                     dangling=dang if dang.any() else None,
+                                     dangling_mode=dmode, retire="per_column")
+        t_cpu = time.perf_counter() - t0
+        Xref, _, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-13, max_iter=600,
+                                   dangling=dang if dang.any() else None,
+                                   dangling_mode=dmode, retire="per_column", x0=X64)
+        X = res.X[:, torch.as_tensor(cols, device=res.X.device)].cpu().numpy().astype(np.float64)
+        l1 = np.abs(X - Xref).sum(0)
+        mx = np.abs(X - Xref).max(0)
+        out["l1_vs_fp64"] = [float(f"{v:.3e}") for v in l1]
+        out["max_abs_vs_fp64"] = [float(f"{v:.3e}") for v in mx]
+        out["checked_columns"] = cols.tolist()
+        out["cpu_baseline"] = {"value": round(len(cols) / t_cpu, 4), "unit": "PPR/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(cols)} of the {K} seed sets, FP64 scipy power "
+                                         f"iteration to the same tol (oracle/ppr.py power), "
+                                         f"{int(it64.max())} iterations in {t_cpu:.1f} s"}
+        log(f"  FP64 check: L1 {l1}, max abs {mx}; CPU {len(cols)/t_cpu:.3f} PPR/s")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -134,6 +222,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log(f"calibration copies of {args.calibrate_copy_mib} MiB done")
+    if args.ppr_only:
+        print(json.dumps({"ppr": ppr_leg(args, dev)}), flush=True)
+        return
     log("generating the 1M x 1M SpMM matrix ...")
     A = graphs.spmm_matrix(seed=0)
     log(f"  nnz={A.nnz}")
@@ -184,6 +275,9 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(A, args.k, args.cpu_seconds)
+    del mats, pbr, A
+    torch.cuda.empty_cache()
+    ppr = None if (args.no_ppr or args.only_headline) else ppr_leg(args, dev)
     line = {
         "metric": "SpMM GNZ/s per RHS (1M x 1M PBR 8x8, ~27M nnz)",
         "value": round(gnz, 2),
@@ -206,6 +300,7 @@ def main():
         "cpu_baseline": cpu,
         "per_rhs_width": per_k,
         "builder": {"ms": round(t_build * 1e3, 2), "note": "includes host->device upload"},
+        "ppr": ppr,
     }
     print(json.dumps(line), flush=True)
 

@@ -95,12 +95,17 @@ __global__ void k_ctl_init(PprCtl* ctl, ColState c, int64_t K, float alpha, int6
 
 // After each pass: sum the per-workgroup partials of every column (double), record the
 // residual, freeze converged columns (or all when retiring together), stop everything at
-// max_iter, and refresh the dangling coefficients for the next pass. One workgroup.
+// max_iter, and refresh the dangling coefficients for the next pass. One workgroup:
+// consecutive threads take consecutive columns (coalesced rows of the partial arrays) and
+// kRetireThreads / KC slices of threads split the workgroup rows; an LDS tree in fixed order
+// combines the slices, so the sums are deterministic.
 __global__ void __launch_bounds__(kRetireThreads)
 k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ part_res,
          const float* __restrict__ part_dang, float* __restrict__ hist, float tol, float alpha,
          int64_t N, int dmode, int together, int max_iter) {
   __shared__ float s_res[kMaxK];
+  __shared__ double s_r[kRetireThreads];
+  __shared__ double s_d[kRetireThreads];
   __shared__ int s_max_bits;
   __shared__ int s_active;
   if (ctl->active <= 0) return;  // the pass was skipped: nothing to retire
@@ -109,33 +114,41 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
     s_max_bits = 0;
     s_active = 0;
   }
-  constexpr int TPC = 8;  // threads per column
-  const int sub = threadIdx.x % TPC;
+  int KC = 64;  // columns per sweep: smallest power of two >= K, at most 64
+  while (KC > 1 && KC / 2 >= K) KC /= 2;
+  const int S = kRetireThreads / KC;  // slices
+  const int kk = threadIdx.x % KC;
+  const int sl = threadIdx.x / KC;
   __syncthreads();
-  for (int64_t k0 = 0; k0 < K; k0 += kRetireThreads / TPC) {
-    const int64_t k = k0 + threadIdx.x / TPC;
+  for (int64_t k0 = 0; k0 < K; k0 += KC) {
+    const int64_t k = k0 + kk;
     double r = 0.0, dg = 0.0;
     if (k < K && !c.frozen[k]) {
-      for (int w = sub; w < nwg; w += TPC) {
+      for (int w = sl; w < nwg; w += S) {
         r += part_res[(int64_t)w * K + k];
         dg += part_dang[(int64_t)w * K + k];
       }
     }
-#pragma unroll
-    for (int m = 1; m < TPC; m <<= 1) {
-      r += __shfl_xor(r, m, kWave);
-      dg += __shfl_xor(dg, m, kWave);
+    s_r[threadIdx.x] = r;
+    s_d[threadIdx.x] = dg;
+    __syncthreads();
+    for (int h = S / 2; h >= 1; h /= 2) {
+      if (sl < h) {
+        s_r[threadIdx.x] += s_r[threadIdx.x + h * KC];
+        s_d[threadIdx.x] += s_d[threadIdx.x + h * KC];
+      }
+      __syncthreads();
     }
-    if (k < K && sub == 0) {
-      const float rf = (float)r;
+    if (sl == 0 && k < K) {
+      const float rf = (float)s_r[kk];
       s_res[k] = rf;
       if (!c.frozen[k]) {
-        c.dsum[k] = (float)dg;
+        c.dsum[k] = (float)s_d[kk];
         atomicMax(&s_max_bits, __float_as_int(rf));  // rf >= 0: int order == float order
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   const float maxres = __int_as_float(s_max_bits);
   const bool last = it + 1 >= max_iter;
   for (int64_t k = threadIdx.x; k < K; k += blockDim.x) {

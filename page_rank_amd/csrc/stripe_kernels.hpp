@@ -32,6 +32,24 @@
 #ifndef GL_BIT_TILE_U
 #define GL_BIT_TILE_U 4
 #endif
+// Cache policy of the loads that stream the matrix (each byte read once per pass): 0 =
+// default, 2 = NT (non-temporal). Measured on MI355X, NT was 3-13 % slower (SpMM and the
+// PPR step), so the default policy stays.
+#ifndef GL_STREAM_CP
+#define GL_STREAM_CP 0
+#endif
+// Remainder entries whose X-row gathers are issued together (feature-lane kernel).
+#ifndef GL_REM_EB
+#define GL_REM_EB 4
+#endif
+// Minimum waves per SIMD the feature-lane kernel is compiled for (caps its VGPRs); 0 = free.
+#ifndef GL_FEAT_WAVES
+#define GL_FEAT_WAVES 0
+#endif
+// Non-temporal stores of finished output rows (1 = on; measured slower, off by default).
+#ifndef GL_NT_STORE
+#define GL_NT_STORE 0
+#endif
 
 
 namespace gl {
@@ -68,17 +86,18 @@ struct RhsView {
 constexpr uint32_t kOob = 0x80000000u;        // never inside a descriptor (< 2 GB)
 constexpr int64_t kBufLimit = 0x7fff0000ll;    // largest array (bytes) for Ld<true>
 
-template <bool BUF>
+template <bool BUF, int CP = 0>
 struct Ld;
 
-template <>
-struct Ld<true> {
+// CP: cache-policy bits of every load (0 = default, 2 = non-temporal).
+template <int CP>
+struct Ld<true, CP> {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ Ld(const void* p, int64_t bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
                                             0x00020000)) {}
   __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, ok ? (uint32_t)i * 4u : kOob, sb, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, ok ? (uint32_t)i * 4u : kOob, sb, CP);
   }
   __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
     return __builtin_bit_cast(float, u32(i, ok, sb));
@@ -87,32 +106,37 @@ struct Ld<true> {
   // into buffer_load_dwordx2/x4); the vector-result buffer builtins are not used.
   __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
     const uint32_t o = ok ? (uint32_t)i * 8u : kOob;
-    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0);
-    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0);
+    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, CP);
+    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, CP);
     return ((uint64_t)hi << 32) | lo;
   }
   __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
     const uint32_t o = ok ? (uint32_t)i * 4u : kOob;
     return make_float4(
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 8u, 0, 0)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 12u, 0, 0)));
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, CP)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, CP)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 8u, 0, CP)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 12u, 0, CP)));
   }
 };
 
-template <>
-struct Ld<false> {
+template <int CP>
+struct Ld<false, CP> {
   const char* p;
   __device__ __forceinline__ Ld(const void* q, int64_t) : p((const char*)q) {}
+  template <class T>
+  __device__ __forceinline__ T get(const char* a) const {
+    if constexpr (CP != 0) return __builtin_nontemporal_load((const T*)a);
+    else return *(const T*)a;
+  }
   __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return ok ? *(const uint32_t*)(p + i * 4 + sb) : 0u;
+    return ok ? get<uint32_t>(p + i * 4 + sb) : 0u;
   }
   __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return ok ? *(const float*)(p + i * 4 + sb) : 0.f;
+    return ok ? get<float>(p + i * 4 + sb) : 0.f;
   }
   __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
-    return ok ? *(const uint64_t*)(p + i * 8) : 0ull;
+    return ok ? get<uint64_t>(p + i * 8) : 0ull;
   }
   __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
     return ok ? *(const float4*)(p + i * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -122,7 +146,8 @@ struct Ld<false> {
 // The arrays of one SpMM as predicated loaders.
 template <bool BUF>
 struct PbrLd {
-  Ld<BUF> codes, coords, offsets, data, rptr, rcol, rval, x;
+  Ld<BUF, GL_STREAM_CP> codes, coords, offsets, data, rptr, rcol, rval;
+  Ld<BUF> x;
   __device__ __forceinline__ PbrLd(const struct PbrView& m, const RhsView& d);
 };
 
@@ -232,7 +257,9 @@ struct EpiStore {
   __device__ __forceinline__ void begin_stripe(int64_t) {}
   __device__ __forceinline__ void put(int64_t row, int64_t col, float acc) {
     float* p = y + elem(row, ldy, col, u24);
-    *p = accumulate ? (*p + acc) : acc;
+    if (accumulate) *p += acc;
+    else if (GL_NT_STORE) __builtin_nontemporal_store(acc, p);
+    else *p = acc;
   }
   __device__ __forceinline__ void finish(int, float*) {}
 };
@@ -294,7 +321,8 @@ struct EpiPpr {
       res += fabsf(xn - xc);
       if (dmask != nullptr && dmask[row]) dang += xn;
     }
-    xnext[ix] = xn;
+    if (GL_NT_STORE) __builtin_nontemporal_store(xn, xnext + ix);
+    else xnext[ix] = xn;
   }
   // slot: the lane's column minus k0 (or -1); red: 2*64 floats of shared memory.
   __device__ __forceinline__ void finish(int slot, float* red) {
@@ -443,14 +471,18 @@ __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const Pbr
 
 // Feature-lane kernel, KW in {16, 32, 64}.
 template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
-__global__ void __launch_bounds__(256) k_stripe_feat(PbrView m, RhsView d, Epi epi) {
+__global__ void __launch_bounds__(256)
+#if GL_FEAT_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(GL_FEAT_WAVES)))
+#endif
+k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int G = kWave / KW;           // tiles in flight per wave
   constexpr int P = B * B;                // bits per tile
   constexpr int NV = (P + 15) / 16;       // value registers per lane
   constexpr int U = GL_FEAT_TILE_U;       // tile groups whose loads are issued together
   constexpr int L = kWave / B;            // remainder: lanes per row
   constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // remainder: RHS columns per lane
-  constexpr int EB = L < 4 ? L : 4;       // remainder: entries whose X loads go together
+  constexpr int EB = L < GL_REM_EB ? L : GL_REM_EB;  // remainder: X gathers issued together
   __shared__ float red[128];
   __shared__ float racc[4][B * KW];
 
