@@ -231,6 +231,38 @@ int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, 
 /* Scale every column of x [N][K] to sum 1 (sums: K floats of scratch). */
 int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * Column normalization of a transition matrix A[target, source] given as CSR on the device
+ * (entries in CSR order: rows ascending, columns ascending inside a row). Replaces
+ * normalize_transition_matrix (Backend/graph_link/graph_link/__init__.py:184-207) without its
+ * dense 1/N block: out_vals[e] = vals[e] * (1 / colsum[indices[e]]), with the float32 column
+ * sums taken in the same order as the host's numpy reduction (bit-identical output when no
+ * column is dangling); dangling[c] = 1 for a zero column sum (col_sums and dangling may be
+ * NULL). The matrix keeps its structure; dangling columns are then handled inside the PPR
+ * run (GL_DANGLING_UNIFORM with this mask) instead of densified.
+ * ---------------------------------------------------------------------------------- */
+int64_t gl_csr_normalize_workspace_bytes(int64_t cols, int64_t nnz);
+int gl_csr_normalize_columns_f32(int64_t cols, int64_t nnz, const int32_t* indices,
+                                 const float* vals, float* out_vals, float* col_sums,
+                                 uint8_t* dangling, void* workspace, int64_t workspace_bytes,
+                                 void* stream);
+
+/* PBR -> canonical CSR on the device (tiles in any order, remainder rows in any order).
+ * Replaces pbr_to_csr (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:289-338, bound at
+ * Backend/graph_link/bindings/bindings.cpp:75-78); unlike it, a last block row that only
+ * partly lies inside the matrix is accepted (its out-of-matrix bits must be clear).
+ * nnz = entries the output arrays hold (indptr[rows+1], indices[nnz], vals[nnz]); the call
+ * fails with GL_ERR_INVALID if the matrix holds a different count, a tile bit lies outside
+ * the matrix, or offsets / remainder indices are out of range. Synchronizes `stream` once. */
+int64_t gl_pbr_to_csr_workspace_bytes(int64_t rows, int64_t nnz);
+int gl_pbr_to_csr_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                          int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                          const uint64_t* codes, const int32_t* coords, const int32_t* offsets,
+                          const float* data, const int32_t* rem_indptr,
+                          const int32_t* rem_col_ind, const float* rem_vals, int32_t* indptr,
+                          int32_t* indices, float* vals, int64_t nnz, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,12 +12,20 @@ from .ops import (SpmmPlan, block_spmm_into, pbr_batched_matmul_cpu, pbr_batched
                   pbr_batched_matmul_triton_gpu, pbr_matmul, spmm_into)
 from .ppr import (PPRResult, normalize_transition_matrix, run_personalized_pagerank,
                   run_ppr)
-from .builder import csr_to_pbr, pbr_to_csr
-from . import graphs, warmstart
+from .builder import csr_to_pbr
+from .csr import (DeviceCSR, load_pbr, normalize_transition_matrix_gpu, pbr_to_csr,
+                  pbr_to_scipy, save_pbr, upload_csr)
+from . import autotune, graphs, warmstart
 
 __all__ = [
     "csr_to_pbr",
     "pbr_to_csr",
+    "pbr_to_scipy",
+    "save_pbr",
+    "load_pbr",
+    "DeviceCSR",
+    "upload_csr",
+    "normalize_transition_matrix_gpu",
     "pbr_matmul",
     "pbr_batched_matmul_cpu",
     "pbr_batched_matmul_cuda",
@@ -35,6 +43,7 @@ __all__ = [
     "PBRMatrixInt64Float",
     "PBRMatrixInt32Double",
     "PBRMatrixInt64Double",
+    "autotune",
     "graphs",
     "warmstart",
 ]

@@ -96,6 +96,13 @@ _PROTOS = {
                                       c_p, c_p]),
     "gl_ws_scatter": (ctypes.c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "gl_ws_normalize": (ctypes.c_int, [c_p, c_i64, c_i64, c_p, c_p]),
+    "gl_csr_normalize_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "gl_csr_normalize_columns_f32": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                    c_i64, c_p]),
+    "gl_pbr_to_csr_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "gl_pbr_to_csr_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64, c_i64,
+                                             c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                             c_i64, c_p, c_i64, c_p]),
 }
 
 _lib = None

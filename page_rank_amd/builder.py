@@ -39,32 +39,51 @@ def _scalar_dtype(mat):
 
 def csr_to_pbr(mat, block_rows: int = 2, block_cols: int = 2, min_nnz_per_block: int = 2, *,
                order: str = "reference", device=None):
-    """Build a PBR matrix from a canonical CSR matrix on the GPU."""
+    """Build a PBR matrix from a canonical CSR matrix on the GPU. `mat` is a scipy CSR, a
+    CPU torch sparse CSR tensor, or a DeviceCSR already on the GPU (no upload)."""
+    from .csr import DeviceCSR
+    if isinstance(mat, DeviceCSR):
+        return _build(mat.shape, mat.nnz, mat.indptr, mat.indices, mat.data,
+                      np.dtype(np.int32), np.dtype(np.float32), block_rows, block_cols,
+                      min_nnz_per_block, order, mat.device, mat.canonical)
     mat = _as_csr_arrays(mat)
     if order not in ("reference", "canonical"):
         raise ValueError("order must be 'reference' or 'canonical'")
     it, st = _index_dtype(mat), _scalar_dtype(mat)
-    cls = pbr_class(it, st)
-    rows, cols = mat.shape
-    if block_rows < 1 or block_cols < 1 or block_rows * block_cols > 64:
-        raise ValueError(f"block {block_rows}x{block_cols} does not fit a 64-bit code")
-    if not 0 <= min_nnz_per_block <= 255:
-        raise ValueError("min_nnz_per_block must be in [0, 255]")
-    if max(rows, cols, mat.nnz) >= 2**31 - 1:
+    if max(mat.shape[0], mat.shape[1], mat.nnz) >= 2**31 - 1:
         raise ValueError("matrix too large for int32 device indices")
     if not mat.has_canonical_format:
         # the builder needs sorted, duplicate-free rows (the reference silently produces
         # misaligned values otherwise)
         mat = mat.copy()
         mat.sum_duplicates()
-    L = _native.lib()
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    vb = 8 if st == np.float64 else 4
+    if dev.type == "cuda" and dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
     with torch.cuda.device(dev):
-        stream = _native.stream_handle(dev)
         indptr = torch.as_tensor(mat.indptr.astype(np.int32, copy=False)).to(dev)
         indices = torch.as_tensor(mat.indices.astype(np.int32, copy=False)).to(dev)
         vals = torch.as_tensor(np.ascontiguousarray(mat.data.astype(st, copy=False))).to(dev)
+    return _build(mat.shape, mat.nnz, indptr, indices, vals, it, st, block_rows, block_cols,
+                  min_nnz_per_block, order, dev, True)
+
+
+def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
+           min_nnz_per_block, order, dev, canonical):
+    if not canonical:
+        raise ValueError("csr_to_pbr needs canonical CSR (sorted, no duplicates)")
+    if order not in ("reference", "canonical"):
+        raise ValueError("order must be 'reference' or 'canonical'")
+    cls = pbr_class(it, st)
+    rows, cols = shape
+    if block_rows < 1 or block_cols < 1 or block_rows * block_cols > 64:
+        raise ValueError(f"block {block_rows}x{block_cols} does not fit a 64-bit code")
+    if not 0 <= min_nnz_per_block <= 255:
+        raise ValueError("min_nnz_per_block must be in [0, 255]")
+    L = _native.lib()
+    vb = 8 if st == np.float64 else 4
+    with torch.cuda.device(dev):
+        stream = _native.stream_handle(dev)
         wsb = L.gl_pbr_build_workspace_bytes(rows, block_rows)
         ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
         rem_indptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
@@ -127,8 +146,4 @@ def csr_to_pbr(mat, block_rows: int = 2, block_cols: int = 2, min_nnz_per_block:
         meta = {"codes": codes, "coords": coords, "offsets": offsets, "data": data,
                 "rem_indptr": rem_indptr, "rem_col_ind": rem_col, "rem_vals": rem_vals}
         meta["stripe_ptr"] = stripe_index(coords, rows, block_rows, T, dev)
-    return cls(rows, cols, block_rows, block_cols, mat.nnz, device_meta=meta)
-
-
-def pbr_to_csr(pbr_mat, indptr, indices, data):
-    raise NotImplementedError("pbr_to_csr is not available yet")
+    return cls(rows, cols, block_rows, block_cols, nnz, device_meta=meta)

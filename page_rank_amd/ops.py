@@ -17,7 +17,9 @@ def _meta_or_raise(pbr_mat):
 
 def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mode="fused",
               kw=0):
-    """y (+)= A @ x for one [rows, K] right-hand side, both on the matrix's device."""
+    """y (+)= A @ x for one [rows, K] right-hand side, both on the matrix's device.
+    kw: RHS columns per launch (0 = library rule, "auto" = measured by the autotuner, which
+    then also picks the mode)."""
     meta = _meta_or_raise(pbr_mat)
     if x.dtype != torch.float32 or y.dtype != torch.float32:
         raise TypeError(f"Unsupported dtype combination: index={meta['coords'].dtype}, "
@@ -28,6 +30,10 @@ def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mo
     if x.stride(1) != 1 or y.stride(1) != 1:
         raise ValueError("x and y must have unit stride along K")
     K = x.shape[1]
+    if kw == "auto":
+        from .autotune import tune_spmm
+        t = tune_spmm(pbr_mat, K)
+        kw, mode = t["kw"], t["mode"]
     L = _native.lib()
     m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
     with torch.cuda.device(x.device):
@@ -128,6 +134,10 @@ class SpmmPlan:
             raise ValueError("shape mismatch")
         if x.stride(1) != 1 or y.stride(1) != 1:
             raise ValueError("x and y must have unit stride along K")
+        if kw == "auto":
+            from .autotune import tune_spmm
+            t = tune_spmm(pbr_mat, x.shape[1])
+            kw, mode = t["kw"], t["mode"]
         self._keep = (meta, x, y)
         self._fn = _native.lib().gl_pbr_spmm_i32_f32
         m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]

@@ -72,7 +72,9 @@ def _seed_arrays(seed_sets, weights=None):
 def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=None,
             dangling_mode="none", retire="per_column", x0=None, weights=None,
             graph_iters=8, kw=0, grid_mult=0, return_history=True) -> PPRResult:
-    """Batched PPR, one column per seed set (a list of node-index lists)."""
+    """Batched PPR, one column per seed set (a list of node-index lists).
+    kw / grid_mult: RHS columns per launch and workgroups per CU (0 = library rule;
+    kw="auto" = measured by the autotuner, cached per shape)."""
     meta = get_pbr_gpu_meta(pbr_mat)
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
@@ -86,6 +88,12 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
     if idx.size and (idx.min() < 0 or idx.max() >= N):
         raise ValueError("seed node out of range")
     K = len(seed_sets)
+    if kw == "auto":
+        from .autotune import tune_ppr
+        t = tune_ppr(pbr_mat, K)
+        kw = t["kw"]
+        if not grid_mult:
+            grid_mult = t["grid_mult"]
     dmode = _DANGLING[dangling_mode]
     dmask = None
     if dmode != _native.GL_DANGLING_NONE:
