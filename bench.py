@@ -173,6 +173,43 @@ def ppr_leg(args, dev):
            "converged": int(res.converged.sum()),
            "algorithmic_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
            "build_ms": round(t_build * 1e3, 1)}
+    wpath = os.path.join(ROOT, "page_rank_amd", "data", "warmstart_clustered.npz")
+    if os.path.exists(wpath):
+        from page_rank_amd import warmstart as ws
+        params = ws.load(wpath, dev)
+        wtimes = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            wi = ws.inputs(pbr, sets, 0.85)
+            x0, _ = ws.predict(params, wi)
+            del wi
+            wres = gl.run_ppr(pbr, sets, x0=x0, **kw)
+            torch.cuda.synchronize()
+            wtimes.append(time.perf_counter() - t0)
+        tw = float(np.median(wtimes))
+        out["learned"] = {"ppr_per_s": round(K / tw, 3), "ms": round(tw * 1e3, 3),
+                          "iterations_max": wres.total_iterations,
+                          "iterations_mean": round(float(wres.iterations.mean()), 2),
+                          "converged": int(wres.converged.sum()),
+                          "speedup_vs_plain": round(t / tw, 3),
+                          "l1_vs_plain_max": float(f"{(wres.X - res.X).abs().sum(0).max().item():.3e}"),
+                          "model": "page_rank_amd/data/warmstart_clustered.npz"}
+        log(f"  learned warm start: {K / tw:.1f} PPR/s, {tw*1e3:.1f} ms, iterations mean "
+            f"{wres.iterations.mean():.1f} ({t / tw:.2f}x plain)")
+        del x0, wres
+    # tolerance that meets L1 < 1e-6 against FP64 (the update-norm stop leaves up to
+    # alpha / (1 - alpha) times tol of error)
+    kw_acc = dict(kw, tol=1.5e-7)
+    gl.run_ppr(pbr, sets, **kw_acc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_acc = gl.run_ppr(pbr, sets, **kw_acc)
+    torch.cuda.synchronize()
+    t_acc = time.perf_counter() - t0
+    out["accurate"] = {"tol": 1.5e-7, "ppr_per_s": round(K / t_acc, 3),
+                       "ms": round(t_acc * 1e3, 3), "iterations_max": res_acc.total_iterations}
+    log(f"  tol 1.5e-7: {K / t_acc:.1f} PPR/s, iterations max {res_acc.total_iterations}")
     if args.ppr_check_cols and not args.no_cpu:
         from oracle import ppr as oppr
         cols = np.random.default_rng(5).choice(K, min(K, args.ppr_check_cols), replace=False)
@@ -185,11 +222,14 @@ def ppr_leg(args, dev):
         Xref, _, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-13, max_iter=600,
                                    dangling=dang if dang.any() else None,
                                    dangling_mode=dmode, retire="per_column", x0=X64)
-        X = res.X[:, torch.as_tensor(cols, device=res.X.device)].cpu().numpy().astype(np.float64)
+        sel = torch.as_tensor(cols, device=res.X.device)
+        X = res.X[:, sel].cpu().numpy().astype(np.float64)
         l1 = np.abs(X - Xref).sum(0)
         mx = np.abs(X - Xref).max(0)
         out["l1_vs_fp64"] = [float(f"{v:.3e}") for v in l1]
         out["max_abs_vs_fp64"] = [float(f"{v:.3e}") for v in mx]
+        Xa = res_acc.X[:, sel].cpu().numpy().astype(np.float64)
+        out["accurate"]["l1_vs_fp64"] = [float(f"{v:.3e}") for v in np.abs(Xa - Xref).sum(0)]
         out["checked_columns"] = cols.tolist()
         out["cpu_baseline"] = {"value": round(len(cols) / t_cpu, 4), "unit": "PPR/s", "cores": 1,
                                "kind": "port",
@@ -275,6 +315,7 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(A, args.k, args.cpu_seconds)
+    nnz = int(A.nnz)
     del mats, pbr, A
     torch.cuda.empty_cache()
     ppr = None if (args.no_ppr or args.only_headline) else ppr_leg(args, dev)
@@ -292,7 +333,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "pbr-spmm-1M-8x8-blockband512-scatter5M", "rows": N,
-                   "nnz": int(A.nnz), "tiles": T, "tile_nnz": V, "remainder_nnz": R,
+                   "nnz": nnz, "tiles": T, "tile_nnz": V, "remainder_nnz": R,
                    "block": 8, "min_nnz_per_block": 4, "K": args.k, "mode": args.mode,
                    "rotating_matrix_copies": len(mats)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,

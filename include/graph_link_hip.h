@@ -263,6 +263,14 @@ int gl_pbr_to_csr_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_
                           int32_t* indices, float* vals, int64_t nnz, void* workspace,
                           int64_t workspace_bytes, void* stream);
 
+/* Top-k of every column of a dense row-major x [N][ld] (K columns): values [k][K] and
+ * rows [k][K] (int64), largest first, equal values by ascending row (deterministic).
+ * Replaces the torch.topk over the rank vector of Tests/wiki_ppr.py:187. */
+int64_t gl_topk_workspace_bytes(int64_t K, int64_t k);
+int gl_topk_columns_f32(const float* x, int64_t N, int64_t K, int64_t ld, int64_t k,
+                        float* values, int64_t* rows, void* workspace, int64_t workspace_bytes,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,8 @@ from .ppr import (PPRResult, normalize_transition_matrix, run_personalized_pager
 from .builder import csr_to_pbr
 from .csr import (DeviceCSR, load_pbr, normalize_transition_matrix_gpu, pbr_to_csr,
                   pbr_to_scipy, save_pbr, upload_csr)
-from . import autotune, graphs, warmstart
+from . import autotune, graphs, ingest, warmstart
+from .ingest import edges_to_adjacency, load_edge_list, top_out_degree_seeds, topk_columns
 
 __all__ = [
     "csr_to_pbr",
@@ -44,6 +45,11 @@ __all__ = [
     "PBRMatrixInt32Double",
     "PBRMatrixInt64Double",
     "autotune",
+    "ingest",
+    "load_edge_list",
+    "edges_to_adjacency",
+    "top_out_degree_seeds",
+    "topk_columns",
     "graphs",
     "warmstart",
 ]

@@ -103,6 +103,9 @@ _PROTOS = {
     "gl_pbr_to_csr_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64, c_i64,
                                              c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
                                              c_i64, c_p, c_i64, c_p]),
+    "gl_topk_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "gl_topk_columns_f32": (ctypes.c_int, [c_p, c_i64, c_i64, c_i64, c_i64, c_p, c_p, c_p, c_i64,
+                                           c_p]),
 }
 
 _lib = None

@@ -38,13 +38,29 @@
 #ifndef GL_STREAM_CP
 #define GL_STREAM_CP 0
 #endif
+// Feature-lane kernel: issue the loads of the next tile batch (and, at a stripe's last
+// batch, of the next stripe's first batch) before the FMAs of the current one. Measured on
+// MI355X: no gain for SpMM, and 25 % slower for the K=64 PPR step (the second batch of
+// registers costs a wave per SIMD), so it is off.
+#ifndef GL_FEAT_PIPE
+#define GL_FEAT_PIPE 0
+#endif
 // Remainder entries whose X-row gathers are issued together (feature-lane kernel).
 #ifndef GL_REM_EB
 #define GL_REM_EB 4
 #endif
+// The same for the 64-column feature-lane kernel (measured: 2 is 5 % faster there, 4 is
+// 12 % faster at 32 columns).
+#ifndef GL_REM_EB64
+#define GL_REM_EB64 2
+#endif
 // Minimum waves per SIMD the feature-lane kernel is compiled for (caps its VGPRs); 0 = free.
 #ifndef GL_FEAT_WAVES
 #define GL_FEAT_WAVES 0
+#endif
+// Minimum waves per SIMD of the 64-column feature-lane kernel (caps its VGPRs).
+#ifndef GL_FEAT64_WAVES
+#define GL_FEAT64_WAVES 4
 #endif
 // Non-temporal stores of finished output rows (1 = on; measured slower, off by default).
 #ifndef GL_NT_STORE
@@ -235,8 +251,9 @@ struct TileFma<2> {
 #undef GL_F
 
 // --------------------------------------------------------------------------------------
-// Epilogues. Each lane of the epilogue owns one (row, column) at a time; put() is called
-// once per output element. begin_stripe() runs (wave-uniform) before the stripe's puts and
+// Epilogues. A lane's output column is fixed for the launch (lane_init, once). At the
+// start of each stripe, begin_stripe() (wave-uniform) and prefetch(i, row, col) for the
+// lane's output elements run; at its end put(i, row, col, acc) once per output element;
 // finish() once per lane at the end with the lane's column slot (or -1).
 // --------------------------------------------------------------------------------------
 // Element offset row * ld + col; `u24` (set by the host when rows and ld are below 2^24 and
@@ -254,8 +271,10 @@ struct EpiStore {
   int32_t accumulate;
   bool u24;
   __device__ __forceinline__ bool active() const { return true; }
+  __device__ __forceinline__ void lane_init(int64_t, bool) {}
   __device__ __forceinline__ void begin_stripe(int64_t) {}
-  __device__ __forceinline__ void put(int64_t row, int64_t col, float acc) {
+  __device__ __forceinline__ void prefetch(int, int64_t, int64_t, bool) {}
+  __device__ __forceinline__ void put(int, int64_t row, int64_t col, float acc) {
     float* p = y + elem(row, ldy, col, u24);
     if (accumulate) *p += acc;
     else if (GL_NT_STORE) __builtin_nontemporal_store(acc, p);
@@ -302,23 +321,36 @@ struct EpiPpr {
   float res;
   float dang;
   int32_t e0, e1;
+  int32_t frz;     // this lane's column is frozen
+  float cuv, tev;  // cu[col], tele[col] of this lane's column
+  float xc[8];     // X_t at this lane's output elements of the current stripe
 
   __device__ __forceinline__ bool active() const { return ctl->active > 0; }
+  // The lane's column is fixed for the whole launch: read its state once.
+  __device__ __forceinline__ void lane_init(int64_t col, bool ok) {
+    frz = ok ? frozen[col] : 1;
+    cuv = ok ? cu[col] : 0.f;
+    tev = ok ? tele[col] : 0.f;
+  }
   __device__ __forceinline__ void begin_stripe(int64_t s) {
     e0 = sptr[s];
     e1 = sptr[s + 1];
   }
-  __device__ __forceinline__ void put(int64_t row, int64_t col, float acc) {
+  // Issued at the start of a stripe, so X_t arrives while the stripe is multiplied.
+  __device__ __forceinline__ void prefetch(int i, int64_t row, int64_t col, bool ok) {
+    xc[i] = ok ? xcur[elem(row, ld, col, u24)] : 0.f;
+  }
+  __device__ __forceinline__ void put(int i, int64_t row, int64_t col, float acc) {
     const int64_t ix = elem(row, ld, col, u24);
-    const float xc = xcur[ix];
-    float xn = xc;
-    if (!frozen[col]) {
+    const float x0 = xc[i];
+    float xn = x0;
+    if (!frz) {
       float w = 0.f;
       for (int e = e0; e < e1; ++e)
         if (srow[e] == row && scol[e] == col) w += sw[e];
-      xn = __builtin_fmaf(alpha, acc, cu[col]);
-      xn = __builtin_fmaf(w, tele[col], xn);
-      res += fabsf(xn - xc);
+      xn = __builtin_fmaf(alpha, acc, cuv);
+      xn = __builtin_fmaf(w, tev, xn);
+      res += fabsf(xn - x0);
       if (dmask != nullptr && dmask[row]) dang += xn;
     }
     if (GL_NT_STORE) __builtin_nontemporal_store(xn, xnext + ix);
@@ -474,6 +506,9 @@ template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
 __global__ void __launch_bounds__(256)
 #if GL_FEAT_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(GL_FEAT_WAVES)))
+#else
+// the 64-column kernel is held to 128 VGPRs (4 waves per SIMD)
+__attribute__((amdgpu_waves_per_eu(KW == 64 ? GL_FEAT64_WAVES : 1)))
 #endif
 k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int G = kWave / KW;           // tiles in flight per wave
@@ -482,7 +517,10 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int U = GL_FEAT_TILE_U;       // tile groups whose loads are issued together
   constexpr int L = kWave / B;            // remainder: lanes per row
   constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // remainder: RHS columns per lane
-  constexpr int EB = L < GL_REM_EB ? L : GL_REM_EB;  // remainder: X gathers issued together
+  // remainder: X gathers issued together (fewer at 64 columns, where each holds 8 floats
+  // per lane and registers decide the occupancy)
+  constexpr int EBW = KW >= 64 ? GL_REM_EB64 : GL_REM_EB;
+  constexpr int EB = L < EBW ? L : EBW;
   __shared__ float red[128];
   __shared__ float racc[4][B * KW];
 
@@ -495,6 +533,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int o = lane & 15;
   const int64_t colk = d.k0 + f;
   const bool fok = colk < d.K;
+  epi.lane_init(colk, fok);
   const int kvalid = (int)min<int64_t>(KW, d.K - d.k0);
   const bool vec = (d.ldx & 3) == 0 && kvalid == KW;  // float4 rows, full column chunk
   const uint32_t ldb = (uint32_t)d.ldx * 4u;  // bytes per X row
@@ -504,6 +543,45 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int nvq = kvalid - kq;
   const int gbase = lane & ~(L - 1);
   float* __restrict__ ra = racc[wid];
+
+  // Values and X rows of one batch of G*U tiles (U groups of G tiles, one tile per group).
+  struct FeatBatch {
+    float v[U][NV];
+    float x[U][B];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[u][j] = 0.f;
+#pragma unroll
+        for (int c = 0; c < B; ++c) x[u][c] = 0.f;
+      }
+    }
+  };
+  // Batch starting at tile tb of the 64-tile chunk [tc, tce) whose metadata is in `meta`.
+  auto load_batch = [&](FeatBatch& bt, const TileMeta& meta, int tc, int tb, int tce) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int src = (tb - tc + u * G + g) & (kWave - 1);
+      const bool tv = tb + u * G + g < tce;
+      const uint32_t clo = __shfl(meta.clo, src, kWave);
+      const uint32_t chi = __shfl(meta.chi, src, kWave);
+      const int c0 = __shfl(meta.c0, src, kWave);
+      const int off = __shfl(meta.off, src, kWave);
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int p = 16 * j + o;
+        const bool has = tv && p < P && code_bit(clo, chi, p);
+        bt.v[u][j] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+      }
+      const int64_t xb = xrow<BUF>(c0, d.ldx) + f;
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        const bool okc = tv && fok && (BUF || (int64_t)c0 + c < m.cols);
+        bt.x[u][c] = ld.x.f32(xb, okc, (uint32_t)c * ldb);
+      }
+    }
+  };
 
   const Sched sc = xcd_sched(m.num_stripes);
   int64_t s = sc.first;
@@ -520,11 +598,20 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
   if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rr);
+#if GL_FEAT_PIPE
+  FeatBatch Pb;
+  if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
+  else Pb.clear();
+#endif
 
   for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
     const int64_t sn = s + sc.stride;
     const int64_t snn = sn + sc.stride;
+    epi.begin_stripe(s);
+#pragma unroll
+    for (int i = 0; i < B / G; ++i)
+      epi.prefetch(i, r0 + i * G + g, colk, fok && r0 + i * G + g < m.rows);
     // ---- prefetch for the next stripe
     TileMeta mtn{0u, 0u, 0, 0};
     int tc0 = 0, tc1 = 0;
@@ -549,37 +636,38 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     for (int r = 0; r < B; ++r) acc[r] = 0.f;
 
     if constexpr (TILES) {
+#if GL_FEAT_PIPE
+      // Pb holds the loads of this stripe's first batch, issued during the previous stripe.
+      for (int tc = ta0; tc < ta1; tc += kWave) {
+        const int tce = min(ta1, tc + kWave);
+        if (tc != ta0) {  // stripes of more than 64 tiles: next chunk starts unpipelined
+          mt = load_tile_meta(ld, tc, ta1, lane);
+          load_batch(Pb, mt, tc, tc, tce);
+        }
+        for (int tb = tc; tb < tce; tb += G * U) {
+          const int nb = tb + G * U;
+          FeatBatch Qb;
+          if (nb < tce) load_batch(Qb, mt, tc, nb, tce);
+          else if (tce == ta1 && tb0 < tb1) load_batch(Qb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
+          else Qb.clear();
+#pragma unroll
+          for (int u = 0; u < U; ++u) TileFma<B>::run(acc, Pb.v[u], Pb.x[u]);
+          Pb = Qb;
+        }
+      }
+      if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
+#else
       for (int tc = ta0; tc < ta1; tc += kWave) {
         if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
         const int tce = min(ta1, tc + kWave);
         for (int tb = tc; tb < tce; tb += G * U) {
-          float v[U][NV];
-          float x[U][B];
+          FeatBatch P1;
+          load_batch(P1, mt, tc, tb, tce);
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int src = (tb - tc + u * G + g) & (kWave - 1);
-            const bool tv = tb + u * G + g < tce;
-            const uint32_t clo = __shfl(mt.clo, src, kWave);
-            const uint32_t chi = __shfl(mt.chi, src, kWave);
-            const int c0 = __shfl(mt.c0, src, kWave);
-            const int off = __shfl(mt.off, src, kWave);
-#pragma unroll
-            for (int j = 0; j < NV; ++j) {
-              const int p = 16 * j + o;
-              const bool has = tv && p < P && code_bit(clo, chi, p);
-              v[u][j] = ld.data.f32(off + code_prefix(clo, chi, p), has);
-            }
-            const int64_t xb = xrow<BUF>(c0, d.ldx) + f;
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-              const bool okc = tv && fok && (BUF || (int64_t)c0 + c < m.cols);
-              x[u][c] = ld.x.f32(xb, okc, (uint32_t)c * ldb);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < U; ++u) TileFma<B>::run(acc, v[u], x[u]);
+          for (int u = 0; u < U; ++u) TileFma<B>::run(acc, P1.v[u], P1.x[u]);
         }
       }
+#endif
     }
 
     if constexpr (REM) {
@@ -634,12 +722,11 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       for (int r = 0; r < B; ++r) acc[r] = xor_sum<16>(acc[r]);
     }
 
-    epi.begin_stripe(s);
     if (fok) {
 #pragma unroll
       for (int r = 0; r < B; ++r) {
         const int64_t row = r0 + r;
-        if ((r % G) == g && row < m.rows) epi.put(row, colk, acc[r]);
+        if ((r % G) == g && row < m.rows) epi.put(r / G, row, colk, acc[r]);
       }
     }
     // ---- rotate the pipeline
@@ -674,6 +761,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   const int q = g * B + cl;         // slot among the L lanes of row rl
   const int kwv = (int)min<int64_t>(KW, d.K - d.k0);  // valid columns of this launch
   const bool vec = (d.ldx & 3) == 0 && kwv == KW;  // float4 rows, full column chunk
+  epi.lane_init(d.k0 + q, q < kwv);
 
   const Sched sc = xcd_sched(m.num_stripes);
   int64_t s = sc.first;
@@ -695,6 +783,8 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     const int64_t r0 = s * B;
     const int64_t sn = s + sc.stride;
     const int64_t snn = sn + sc.stride;
+    epi.begin_stripe(s);
+    epi.prefetch(0, r0 + rl, d.k0 + q, q < kwv && r0 + rl < m.rows);
     TileMeta mtn{0u, 0u, 0, 0};
     int tc0 = 0, tc1 = 0;
     RowSpan rsn{0, 0};
@@ -775,14 +865,13 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       acc[k] = t;
     }
 
-    epi.begin_stripe(s);
     const int64_t row = r0 + rl;
     if (q < kwv && row < m.rows) {
       float mine = acc[0];
 #pragma unroll
       for (int k = 1; k < KW; ++k)
         if (q == k) mine = acc[k];
-      epi.put(row, d.k0 + q, mine);
+      epi.put(0, row, d.k0 + q, mine);
     }
     ta0 = tb0;
     ta1 = tb1;

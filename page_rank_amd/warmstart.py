@@ -157,10 +157,12 @@ def predict(params, wi: WsInputs, near_cap=None):
     return x0, n_near
 
 
-def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0):
-    """Training set of one graph: every pair within two hops plus `far_samples` uniformly
-    drawn pairs (weight N*K / far_samples, which makes the weighted loss an unbiased
-    estimate of the full-column L1). Targets are N * x*."""
+def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0,
+                   near_samples=None):
+    """Training set of one graph: the pairs within two hops (all of them, or
+    `near_samples` drawn uniformly with weight n_near / near_samples) plus `far_samples`
+    uniformly drawn pairs beyond (weight N*K / far_samples), which makes the weighted loss
+    an unbiased estimate of the full-column L1. Targets are N * x*."""
     L = _lib()
     dev = wi.v[0].device
     st = _native.stream_handle(dev)
@@ -176,15 +178,22 @@ def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0)
     n_near = min(int(cnt.item()), cap)
     del x0
     g = torch.Generator(device="cpu").manual_seed(seed)
+    near = near[:n_near]
+    w_near = 1.0
+    if near_samples is not None and n_near > near_samples:
+        pick = torch.randperm(n_near, generator=g)[:near_samples].to(dev)
+        near = near[pick]
+        w_near = n_near / near_samples
+        n_near = near_samples
     far = torch.randint(0, wi.N * wi.K, (far_samples,), generator=g).to(dev)
-    pairs = torch.cat([near[:n_near], far])
+    pairs = torch.cat([near, far])
     feat = pair_features(wi, pairs)
     # drop far draws that landed within two hops (they are already in the near list)
     hop_far = feat[n_near:, 2:5].sum(1) == 0
     keep = torch.cat([torch.ones(n_near, dtype=torch.bool, device=dev), hop_far])
     pairs, feat = pairs[keep], feat[keep]
     n_far = int(hop_far.sum().item())
-    w = torch.ones(pairs.numel(), dtype=torch.float32, device=dev)
+    w = torch.full((pairs.numel(),), float(w_near), dtype=torch.float32, device=dev)
     w[n_near:] = float(wi.N * wi.K) / far_samples
     target = x_star.reshape(-1)[pairs] * wi.N
     return feat, target, w, n_near, n_far
@@ -251,8 +260,8 @@ def load(path, device="cuda"):
         return torch.as_tensor(z["params"], device=device)
 
 
-def fit(samples, params=None, *, alpha=0.85, far_samples=1 << 18, epochs=8, batch=1 << 16,
-        lr=2e-3, tol=1e-7, max_iter=400, seed=0):
+def fit(samples, params=None, *, alpha=0.85, far_samples=1 << 18, near_samples=None, epochs=8,
+        batch=1 << 16, lr=2e-3, tol=1e-7, max_iter=400, seed=0):
     """Train on a list of (pbr, seed_sets, dangling_mask, dangling_mode) graphs: the
     exact solution of each is computed by run_ppr, then pairs are drawn by
     training_pairs and the network is fitted with train. Returns (params, TrainLog)."""
@@ -264,7 +273,9 @@ def fit(samples, params=None, *, alpha=0.85, far_samples=1 << 18, epochs=8, batc
         dev = wi.v[0].device
         ref = run_ppr(pbr, sets, alpha=alpha, tol=tol, max_iter=max_iter, dangling=dang,
                       dangling_mode=dmode, return_history=False)
-        f, t, w, _, _ = training_pairs(wi, ref.X, far_samples, seed=seed + j)
+        f, t, w, _, _ = training_pairs(wi, ref.X, far_samples, seed=seed + j,
+                                       near_samples=near_samples)
+        del wi, ref
         feats.append(f), targets.append(t), weights.append(w)
     if params is None:
         params = init_params(seed, dev)
@@ -312,3 +323,80 @@ def evaluate(params, pbr, seed_sets, dangling=None, dangling_mode="none", *, alp
         plain.iterations, warm.iterations, (t1 - t0) * 1e3, (t3 - t1) * 1e3, (t2 - t1) * 1e3,
         (P - x_star).abs().sum(0).cpu().numpy(), (x0 - x_star).abs().sum(0).cpu().numpy(),
         float((plain.X - warm.X).abs().sum(0).max().item()))
+
+
+def clustered_sample(n, m, sets, seed, block=8, min_nnz=4):
+    """One graph of the block-clustered generator (communities of 1,000 nodes) with `sets`
+    seed sets of U{1..100} nodes, built on the GPU: (pbr, seed_sets, dangling, mode)."""
+    from . import graphs
+    from .builder import csr_to_pbr
+    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000), seed=seed)
+    A, dang = graphs.transition_matrix(src, dst, n)
+    del src, dst
+    pbr = csr_to_pbr(A, block, block, min_nnz, order="canonical")
+    return (pbr, graphs.seed_sets(n, sets, 1, 100, seed=seed + 100000),
+            dang if dang.any() else None, "uniform" if dang.any() else "none")
+
+
+def main(argv=None):
+    """Train on seeded clustered graphs and evaluate on held-out ones:
+
+        python -m page_rank_amd.warmstart --graphs 20 --nodes 1000000 --edges 16000000 \
+            --sets 100 --out page_rank_amd/data/warmstart_clustered.npz
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--graphs", type=int, default=20)
+    ap.add_argument("--heldout", type=int, default=2)
+    ap.add_argument("--nodes", type=int, default=1_000_000)
+    ap.add_argument("--edges", type=int, default=16_000_000)
+    ap.add_argument("--sets", type=int, default=100)
+    ap.add_argument("--far-samples", type=int, default=1 << 18)
+    ap.add_argument("--near-samples", type=int, default=1 << 18)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    feats, targets, weights = [], [], []
+    dev = None
+    for g in range(a.graphs):
+        pbr, sets, dang, dmode = clustered_sample(a.nodes, a.edges, a.sets, seed=1000 + g)
+        wi = inputs(pbr, sets)
+        dev = wi.v[0].device
+        from .ppr import run_ppr
+        ref = run_ppr(pbr, sets, tol=1e-7, max_iter=400, dangling=dang, dangling_mode=dmode,
+                      return_history=False)
+        f, t, w, n_near, n_far = training_pairs(wi, ref.X, a.far_samples, seed=g,
+                                                near_samples=a.near_samples)
+        feats.append(f), targets.append(t), weights.append(w)
+        print(f"graph {g}: {n_near} near + {n_far} far pairs, "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+        del wi, ref, pbr
+    params = init_params(0, dev)
+    log = train(params, torch.cat(feats), torch.cat(targets), torch.cat(weights),
+                epochs=a.epochs, batch=1 << 16, lr=2e-3)
+    print(f"trained in {log.seconds:.1f} s, losses {[round(v, 4) for v in log.losses]}",
+          flush=True)
+    del feats, targets, weights
+    if a.out:
+        save(params, a.out)
+    evals = []
+    for h in range(a.heldout):
+        pbr, sets, dang, dmode = clustered_sample(a.nodes, a.edges, a.sets, seed=5000 + h)
+        ev = evaluate(params, pbr, sets, dang, dmode, tol=1e-6)
+        evals.append({"plain_iters_mean": float(ev.plain_iters.mean()),
+                      "warm_iters_mean": float(ev.warm_iters.mean()),
+                      "plain_ms": round(ev.plain_ms, 3), "warm_ms": round(ev.warm_ms, 3),
+                      "predict_ms": round(ev.predict_ms, 3),
+                      "start_l1_plain": float(ev.start_l1_plain.mean()),
+                      "start_l1_warm": float(ev.start_l1_warm.mean()),
+                      "max_l1_diff": ev.max_l1_diff})
+        print(json.dumps(evals[-1]), flush=True)
+    print(json.dumps({"graphs": a.graphs, "nodes": a.nodes, "edges": a.edges, "sets": a.sets,
+                      "train_seconds": round(log.seconds, 2), "losses": log.losses,
+                      "heldout": evals}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
