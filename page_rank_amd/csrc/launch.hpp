@@ -17,12 +17,32 @@ template <int B, class Epi, bool TILES, bool REM, bool BUF>
 int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
                      hipStream_t st) {
   const dim3 grid(nwg), block(256);
+  // tile batch: a whole stripe at once when 8x8 stripes hold about 8 tiles (measured on
+  // MI355X: -16 % at 1 column, -3 % at 32; narrower batches win on sparser tilings)
+  const bool wide = B == 8 && m.num_tiles >= 6 * m.num_stripes;
   switch (kw) {
-#define GL_K(KW, KER) \
-    case KW: hipLaunchKernelGGL((KER<B, KW, TILES, REM, BUF, Epi>), grid, block, 0, st, m, d, e); break;
-    GL_K(1, k_stripe_bit) GL_K(2, k_stripe_bit) GL_K(4, k_stripe_bit) GL_K(8, k_stripe_bit)
-    GL_K(16, k_stripe_feat) GL_K(32, k_stripe_feat) GL_K(64, k_stripe_feat)
-#undef GL_K
+#define GL_KB(KW)                                                                             \
+    case KW:                                                                                  \
+      if (wide)                                                                               \
+        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, (B == 8 ? 8 : 2)>),     \
+                           grid, block, 0, st, m, d, e);                                      \
+      else                                                                                    \
+        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, (B == 8 ? 4 : 2)>),     \
+                           grid, block, 0, st, m, d, e);                                      \
+      break;
+#define GL_KF(KW)                                                                             \
+    case KW:                                                                                  \
+      if (wide)                                                                               \
+        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, 4>), grid, block, 0, st, \
+                           m, d, e);                                                          \
+      else                                                                                    \
+        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, 1>), grid, block, 0, st, \
+                           m, d, e);                                                          \
+      break;
+    GL_KB(1) GL_KB(2) GL_KB(4) GL_KB(8)
+    GL_KF(16) GL_KF(32) GL_KF(64)
+#undef GL_KB
+#undef GL_KF
     default:
       set_error("unsupported kw %d", kw);
       return GL_ERR_INVALID;

@@ -25,12 +25,11 @@
 #ifndef GL_FUSED_DPP_FMA
 #define GL_FUSED_DPP_FMA 1
 #endif
-// tile groups whose loads are issued together (feature-lane / bit-lane kernels)
-#ifndef GL_FEAT_TILE_U
-#define GL_FEAT_TILE_U 2
-#endif
-#ifndef GL_BIT_TILE_U
-#define GL_BIT_TILE_U 4
+// Bit-lane kernel: issue the loads of the next tile batch (and, at a stripe's last batch,
+// of the next stripe's first batch) before the FMAs of the current one. Measured on MI355X:
+// 3-12 % slower for SpMM and 48 % slower for the 8-column PPR step, so it is off.
+#ifndef GL_BIT_PIPE
+#define GL_BIT_PIPE 0
 #endif
 // Cache policy of the loads that stream the matrix (each byte read once per pass): 0 =
 // default, 2 = NT (non-temporal). Measured on MI355X, NT was 3-13 % slower (SpMM and the
@@ -44,6 +43,16 @@
 // registers costs a wave per SIMD), so it is off.
 #ifndef GL_FEAT_PIPE
 #define GL_FEAT_PIPE 0
+#endif
+// Feature-lane remainder: split a stripe's remainder entries (contiguous in the CSR) evenly
+// over lane groups instead of walking its rows in lockstep (1), or row lanes (0).
+// Measured on MI355X: 28 % slower for SpMM at 32 columns and 18 % for the 64-column PPR step
+// (the LDS flushes and row lookups cost more than the balance gains), so it is off.
+#ifndef GL_REM_FLAT
+#define GL_REM_FLAT 0
+#endif
+#ifndef GL_REM_FLAT_EB
+#define GL_REM_FLAT_EB 4
 #endif
 // Remainder entries whose X-row gathers are issued together (feature-lane kernel).
 #ifndef GL_REM_EB
@@ -501,8 +510,9 @@ __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const Pbr
 // about two dependent memory round trips.
 // --------------------------------------------------------------------------------------
 
-// Feature-lane kernel, KW in {16, 32, 64}.
-template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
+// Feature-lane kernel, KW in {16, 32, 64}. UF: tile groups whose loads are issued together
+// (the launcher picks 4 for 8x8 matrices with about 8 tiles per stripe, else 1).
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UF>
 __global__ void __launch_bounds__(256)
 #if GL_FEAT_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(GL_FEAT_WAVES)))
@@ -514,15 +524,20 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int G = kWave / KW;           // tiles in flight per wave
   constexpr int P = B * B;                // bits per tile
   constexpr int NV = (P + 15) / 16;       // value registers per lane
-  constexpr int U = GL_FEAT_TILE_U;       // tile groups whose loads are issued together
+  constexpr int U = UF;
   constexpr int L = kWave / B;            // remainder: lanes per row
   constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // remainder: RHS columns per lane
   // remainder: X gathers issued together (fewer at 64 columns, where each holds 8 floats
   // per lane and registers decide the occupancy)
   constexpr int EBW = KW >= 64 ? GL_REM_EB64 : GL_REM_EB;
   constexpr int EB = L < EBW ? L : EBW;
+  // flat remainder: LE lanes (4 columns each) per entry, NGR entry groups per wave
+  constexpr int LE = KW / 4;
+  constexpr int NGR = kWave / LE;
+  constexpr int EBF = GL_REM_FLAT_EB;
   __shared__ float red[128];
   __shared__ float racc[4][B * KW];
+  __shared__ int rsb_all[4][B + 1];
 
   if (!epi.active()) return;
   const PbrLd<BUF> ld(m, d);
@@ -543,6 +558,10 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int nvq = kvalid - kq;
   const int gbase = lane & ~(L - 1);
   float* __restrict__ ra = racc[wid];
+  int* __restrict__ rsb = rsb_all[wid];
+  const int eg = lane / LE;             // flat remainder: entry group of this lane
+  const int kq4 = (lane % LE) * 4;      // flat remainder: first of this lane's 4 columns
+  const int nv4 = kvalid - kq4;
 
   // Values and X rows of one batch of G*U tiles (U groups of G tiles, one tile per group).
   struct FeatBatch {
@@ -597,7 +616,13 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   TileMeta mt{0u, 0u, 0, 0};
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
+#if GL_REM_FLAT
+  // lane k <= B: remainder start of row r0 + k (clamped to the matrix)
+  int rsv = 0;
+  if (REM && s < sc.end) rsv = (int)ld.rptr.u32(min<int64_t>(s * B + lane, m.rows), lane <= B);
+#else
   if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rr);
+#endif
 #if GL_FEAT_PIPE
   FeatBatch Pb;
   if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
@@ -621,6 +646,27 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       tc0 = m.stripe_ptr[snn];
       tc1 = m.stripe_ptr[snn + 1];
     }
+#if GL_REM_FLAT
+    int rsvn = 0;
+    if (REM && sn < sc.end)
+      rsvn = (int)ld.rptr.u32(min<int64_t>(sn * B + lane, m.rows), lane <= B);
+    // this lane group's chunk of the stripe's remainder entries, and its first batch
+    int cb = 0, ce = 0;
+    int fcol[EBF];
+    float fval[EBF];
+    if constexpr (REM) {
+      const int ea = __shfl(rsv, 0, kWave), ee = __shfl(rsv, B, kWave);
+      const int cs = (ee - ea + NGR - 1) / NGR;
+      cb = min(ee, ea + eg * cs);
+      ce = min(ee, cb + cs);
+#pragma unroll
+      for (int i = 0; i < EBF; ++i) {
+        const bool ok = cb + i < ce;
+        fcol[i] = (int)ld.rcol.u32(cb + i, ok);
+        fval[i] = ld.rval.f32(cb + i, ok);
+      }
+    }
+#else
     if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rr);
     // ---- first remainder entries of this stripe
     int scol = 0;
@@ -630,6 +676,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
       sval = ld.rval.f32(rs.j0 + qq, ok);
     }
+#endif
 
     float acc[B];
 #pragma unroll
@@ -671,6 +718,66 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     }
 
     if constexpr (REM) {
+#if GL_REM_FLAT
+      // Entry groups walk their chunk in order, keep one running sum per lane (4 columns)
+      // and add it into the wave's LDS row accumulator when the row changes. Only this wave
+      // touches its accumulator, so the order of the adds, and the result, is fixed.
+#pragma unroll
+      for (int i = lane; i < B * KW; i += kWave) ra[i] = 0.f;
+      if (lane <= B) rsb[lane] = rsv;
+      wave_lds_sync();
+      int row = 0;
+#pragma unroll
+      for (int k = 1; k < B; ++k) row += rsb[k] <= cb ? 1 : 0;
+      int nxt = rsb[row + 1];
+      float rac[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int e = cb; e < ce; e += EBF) {
+        float xv[EBF][4];
+#pragma unroll
+        for (int i = 0; i < EBF; ++i)
+          load_row<4>(ld.x, xrow<BUF>(fcol[i], d.ldx) + kq4, vec, e + i < ce ? nv4 : 0, xv[i]);
+        float av[EBF];
+#pragma unroll
+        for (int i = 0; i < EBF; ++i) av[i] = fval[i];
+        // the next batch's columns and values travel while this batch's rows do
+#pragma unroll
+        for (int i = 0; i < EBF; ++i) {
+          const bool ok = e + EBF + i < ce;
+          fcol[i] = (int)ld.rcol.u32(e + EBF + i, ok);
+          fval[i] = ld.rval.f32(e + EBF + i, ok);
+        }
+#pragma unroll
+        for (int i = 0; i < EBF; ++i) {
+          const int ei = e + i;
+          if (ei < ce) {
+            if (ei >= nxt) {
+              if (kq4 < KW) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) atomicAdd(&ra[row * KW + kq4 + u], rac[u]);
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) rac[u] = 0.f;
+              do {
+                ++row;
+                nxt = rsb[row + 1];
+              } while (ei >= nxt);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rac[u] = __builtin_fmaf(av[i], xv[i][u], rac[u]);
+          }
+        }
+      }
+      if (cb < ce && kq4 < KW) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) atomicAdd(&ra[row * KW + kq4 + u], rac[u]);
+      }
+      wave_lds_sync();
+      if (g == 0) {
+#pragma unroll
+        for (int r = 0; r < B; ++r) acc[r] += ra[r * KW + f];
+      }
+      wave_lds_sync();
+#else
       // row lanes: the L lanes of row rr hold FPL columns each and walk that row's entries
       float racc2[FPL];
 #pragma unroll
@@ -710,6 +817,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
         for (int r = 0; r < B; ++r) acc[r] += ra[r * KW + f];
       }
       wave_lds_sync();
+#endif
     }
 
     // combine the tile groups: every group ends with the full row sums
@@ -735,7 +843,11 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     tb0 = tc0;
     tb1 = tc1;
     mt = mtn;
+#if GL_REM_FLAT
+    rsv = rsvn;
+#else
     rs = rsn;
+#endif
   }
   epi.finish(fok ? f : -1, red);
 }
@@ -743,11 +855,14 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
 // --------------------------------------------------------------------------------------
 // Bit-lane kernel, KW in {1, 2, 4, 8}.
 // --------------------------------------------------------------------------------------
-template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi>
+// UB: tile groups whose loads are issued together (the launcher picks 8 for 8x8 matrices
+// with about 8 tiles per stripe, where one batch then covers a stripe, else 4; 2 for
+// smaller blocks).
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UB>
 __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi epi) {
   constexpr int P = B * B;
   constexpr int G = kWave / P;      // tiles per wave
-  constexpr int U = B == 8 ? GL_BIT_TILE_U : 2;  // tile groups whose loads go together
+  constexpr int U = UB;
   constexpr int L = kWave / B;      // lanes per matrix row
   __shared__ float red[128];
 
@@ -762,6 +877,36 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   const int kwv = (int)min<int64_t>(KW, d.K - d.k0);  // valid columns of this launch
   const bool vec = (d.ldx & 3) == 0 && kwv == KW;  // float4 rows, full column chunk
   epi.lane_init(d.k0 + q, q < kwv);
+
+  // Value and X row of this lane's bit in each of U tiles.
+  struct BitBatch {
+    float v[U];
+    float x[U][KW];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v[u] = 0.f;
+#pragma unroll
+        for (int k = 0; k < KW; ++k) x[u][k] = 0.f;
+      }
+    }
+  };
+  // Batch starting at tile tb of the 64-tile chunk [tc, tce) whose metadata is in `meta`.
+  auto load_batch = [&](BitBatch& bt, const TileMeta& meta, int tc, int tb, int tce) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int src = (tb - tc + u * G + g) & (kWave - 1);
+      const bool tv = tb + u * G + g < tce;
+      const uint32_t clo = __shfl(meta.clo, src, kWave);
+      const uint32_t chi = __shfl(meta.chi, src, kWave);
+      const int c0 = __shfl(meta.c0, src, kWave);
+      const int off = __shfl(meta.off, src, kWave);
+      const int64_t col = (int64_t)c0 + cl;
+      const bool has = tv && code_bit(clo, chi, p) && (BUF || col < m.cols);
+      bt.v[u] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+      load_row<KW>(ld.x, xrow<BUF>(col, d.ldx), vec, has ? kwv : 0, bt.x[u]);
+    }
+  };
 
   const Sched sc = xcd_sched(m.num_stripes);
   int64_t s = sc.first;
@@ -778,6 +923,11 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
   if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rl);
+#if GL_BIT_PIPE
+  BitBatch Pb;
+  if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
+  else Pb.clear();
+#endif
 
   for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
@@ -808,31 +958,42 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     for (int k = 0; k < KW; ++k) acc[k] = 0.f;
 
     if constexpr (TILES) {
+#if GL_BIT_PIPE
+      // Pb holds the loads of this stripe's first batch, issued during the previous stripe.
+      for (int tc = ta0; tc < ta1; tc += kWave) {
+        const int tce = min(ta1, tc + kWave);
+        if (tc != ta0) {  // stripes of more than 64 tiles: next chunk starts unpipelined
+          mt = load_tile_meta(ld, tc, ta1, lane);
+          load_batch(Pb, mt, tc, tc, tce);
+        }
+        for (int tb = tc; tb < tce; tb += G * U) {
+          const int nb = tb + G * U;
+          BitBatch Qb;
+          if (nb < tce) load_batch(Qb, mt, tc, nb, tce);
+          else if (tce == ta1 && tb0 < tb1) load_batch(Qb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
+          else Qb.clear();
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(Pb.v[u], Pb.x[u][k], acc[k]);
+          Pb = Qb;
+        }
+      }
+      if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
+#else
       for (int tc = ta0; tc < ta1; tc += kWave) {
         if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
         const int tce = min(ta1, tc + kWave);
         for (int tb = tc; tb < tce; tb += G * U) {
-          float v[U];
-          float xr[U][KW];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int src = (tb - tc + u * G + g) & (kWave - 1);
-            const bool tv = tb + u * G + g < tce;
-            const uint32_t clo = __shfl(mt.clo, src, kWave);
-            const uint32_t chi = __shfl(mt.chi, src, kWave);
-            const int c0 = __shfl(mt.c0, src, kWave);
-            const int off = __shfl(mt.off, src, kWave);
-            const int64_t col = (int64_t)c0 + cl;
-            const bool has = tv && code_bit(clo, chi, p) && (BUF || col < m.cols);
-            v[u] = ld.data.f32(off + code_prefix(clo, chi, p), has);
-            load_row<KW>(ld.x, xrow<BUF>(col, d.ldx), vec, has ? kwv : 0, xr[u]);
-          }
+          BitBatch P1;
+          load_batch(P1, mt, tc, tb, tce);
 #pragma unroll
           for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(v[u], xr[u][k], acc[k]);
+            for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(P1.v[u], P1.x[u][k], acc[k]);
         }
       }
+#endif
     }
 
     if constexpr (REM) {
