@@ -316,6 +316,7 @@ def main():
         log("timing the CPU baseline ...")
         cpu = cpu_baseline(A, args.k, args.cpu_seconds)
     nnz = int(A.nnz)
+    n_mats = len(mats)
     del mats, pbr, A
     torch.cuda.empty_cache()
     ppr = None if (args.no_ppr or args.only_headline) else ppr_leg(args, dev)
@@ -335,7 +336,7 @@ def main():
         "config": {"workload": "pbr-spmm-1M-8x8-blockband512-scatter5M", "rows": N,
                    "nnz": nnz, "tiles": T, "tile_nnz": V, "remainder_nnz": R,
                    "block": 8, "min_nnz_per_block": 4, "K": args.k, "mode": args.mode,
-                   "rotating_matrix_copies": len(mats)},
+                   "rotating_matrix_copies": n_mats},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
