@@ -72,6 +72,8 @@ struct FillOut {
   int32_t* blk_bc;    // optional block list for the reference-order replay
   int8_t* blk_rmin;
   int32_t* blk_tile;
+  uint8_t* big;       // stripes left to the merge kernel (more distinct blocks than the
+                      // LDS table holds); NULL = the merge kernel takes every stripe
 };
 
 template <int GL, bool FILL, typename V>
@@ -88,7 +90,8 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
   for (int64_t sb = wave * SPW; sb < a.num_stripes; sb += nwaves * SPW) {
     const int64_t s = sb + gidx;
     const int64_t row = s * a.br + r;
-    const bool lane_ok = s < a.num_stripes && r < a.br && row < a.rows;
+    const bool mine = s < a.num_stripes && (o.big == nullptr || o.big[s] != 0);
+    const bool lane_ok = mine && r < a.br && row < a.rows;
     int32_t p = 0, pe = 0;
     if (lane_ok) {
       p = a.indptr[row];
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
     }
     // pass 2 running offsets (group-uniform values kept in every lane of the group)
     int32_t t_next = 0, v_next = 0, d_next = 0, rem_next = 0;
-    if (FILL && s < a.num_stripes) {
+    if (FILL && mine) {
       t_next = o.nt[s];
       v_next = o.nv[s];
       d_next = o.nd[s];
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
       }
       if (part) p = q;
     }
-    if (!FILL && s < a.num_stripes) {
+    if (!FILL && mine) {
       if (r == 0) {
         o.nt[s] = cnt_t;
         o.nv[s] = cnt_v;
@@ -177,6 +180,286 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
       }
       if (lane_ok) o.nr[row] = cnt_r;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Wave-per-stripe builder. A stripe's entries are one contiguous CSR segment; the wave reads
+// it 64 entries at a time (coalesced) and inserts every entry's block column into an LDS
+// hash table (atomic CAS on the key, atomic OR of the entry's bit into the code, atomic min
+// of its row). The occupied keys are then put in ascending order (rank = number of smaller
+// keys, counted with broadcast LDS reads), which gives
+// the canonical tile order, tile indices and value offsets by prefix sums; a second sweep
+// over the entries writes kept values at popcount(code & bits below) and rejected entries
+// at their rank inside their row (ballot prefix counts), so remainder rows keep column
+// order. Results are the merge kernel's, entry for entry. Stripes with more than kHashCap
+// distinct blocks are flagged and left to the merge kernel.
+// ---------------------------------------------------------------------------------------
+constexpr int kHashCap = 256;
+
+struct HashLds {
+  int32_t key[kHashCap];
+  uint32_t lo[kHashCap], hi[kHashCap];
+  int32_t rmin[kHashCap];
+  int32_t tile[kHashCap];  // per slot: tile index (fill) or 1 (count) when kept, -1 if not
+  int32_t voff[kHashCap];  // per slot: offset of the tile's first value
+  int32_t skey[kHashCap];  // sorted keys / their slots
+  int32_t sslot[kHashCap];
+  int32_t rows[65];        // CSR start of each row of the stripe (block_rows + 1 values)
+  int32_t rcnt[64];        // rejected entries per row so far
+  int32_t flag;            // table overflow
+};
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int hash_slot(int32_t b) {
+  return (int)(((uint32_t)b * 2654435761u) >> 24);  // kHashCap = 256
+}
+// Row (inside the stripe) of entry e: the last k < br with rows[k] <= e.
+__device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
+  int lo = 0, hi = br - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rows[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool FILL, typename V>
+__global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
+  __shared__ HashLds sh_all[4];
+  const int lane = threadIdx.x & (kWave - 1);
+  HashLds& h = sh_all[threadIdx.x / kWave];
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  const V* __restrict__ vals = (const V*)o.values;
+  const int br = a.br, bc = a.bc;
+  const uint64_t below_lane = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+
+  for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
+    const int64_t r0 = s * br;
+    for (int k = lane; k <= br; k += kWave) h.rows[k] = a.indptr[min<int64_t>(r0 + k, a.rows)];
+    if (lane < br) h.rcnt[lane] = 0;
+    for (int i = lane; i < kHashCap; i += kWave) {
+      h.key[i] = -1;
+      h.lo[i] = 0u;
+      h.hi[i] = 0u;
+      h.rmin[i] = INT32_MAX;
+    }
+    if (lane == 0) h.flag = 0;
+    wsync();
+    const int32_t E0 = h.rows[0], E1 = h.rows[br];
+    // row starts 1..7 in registers for the common block_rows <= 8 (a row is then found by
+    // compares instead of a search in LDS)
+    int32_t rs8[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) rs8[k] = k + 1 < br ? h.rows[k + 1] : INT32_MAX;
+    auto row_at = [&](int32_t e) {
+      if (br <= 8) {
+        int r = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) r += rs8[k] <= e ? 1 : 0;
+        return r;
+      }
+      return row_of(h.rows, br, e);
+    };
+
+    // ---- insert every entry's block
+    for (int32_t c = E0; c < E1; c += kWave) {
+      const int32_t e = c + lane;
+      if (e < E1) {
+        const int32_t col = a.indices[e];
+        const int r = row_at(e);
+        const int32_t b = col / bc;
+        const int bit = r * bc + (col - b * bc);
+        int slot = hash_slot(b);
+        int probes = 0;
+        for (;;) {
+          const int32_t old = atomicCAS(&h.key[slot], -1, b);
+          if (old == -1 || old == b) break;
+          slot = (slot + 1) & (kHashCap - 1);
+          if (++probes == kHashCap) {
+            slot = -1;
+            break;
+          }
+        }
+        if (slot < 0) {
+          h.flag = 1;
+        } else {
+          if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
+          else atomicOr(&h.hi[slot], 1u << (bit - 32));
+          atomicMin(&h.rmin[slot], r);
+        }
+      }
+    }
+    wsync();
+    if (h.flag) {
+      if (!FILL && lane == 0) o.big[s] = 1;  // the merge kernel takes this stripe
+      wsync();
+      continue;
+    }
+
+    if constexpr (!FILL) {
+      // counts do not depend on the tile order: sweep the table as it is
+      int32_t nd = 0, nt = 0, nv = 0;
+#pragma unroll
+      for (int t = 0; t < kHashCap / kWave; ++t) {
+        const int i = t * kWave + lane;
+        const bool occ = h.key[i] >= 0;
+        const int32_t cnt = __popc(h.lo[i]) + __popc(h.hi[i]);
+        const bool kept = occ && cnt >= a.min_nnz;
+        h.tile[i] = kept ? 1 : -1;
+        nd += occ ? 1 : 0;
+        nt += kept ? 1 : 0;
+        nv += kept ? cnt : 0;
+      }
+#pragma unroll
+      for (int m = 1; m < kWave; m <<= 1) {
+        nd += __shfl_xor(nd, m, kWave);
+        nt += __shfl_xor(nt, m, kWave);
+        nv += __shfl_xor(nv, m, kWave);
+      }
+      if (lane == 0) {
+        o.nt[s] = nt;
+        o.nv[s] = nv;
+        o.nd[s] = nd;
+      }
+    } else {
+    // ---- occupied keys, compacted, then sorted ascending
+    int32_t D = 0;
+#pragma unroll
+    for (int t = 0; t < kHashCap / kWave; ++t) {
+      const int i = t * kWave + lane;
+      const int32_t k = h.key[i];
+      const uint64_t m = __ballot(k >= 0);
+      if (k >= 0) {
+        const int pos = D + __popcll(m & below_lane);
+        h.skey[pos] = k;
+        h.sslot[pos] = i;
+      }
+      D += __popcll(m);
+    }
+    // rank of each key = number of smaller keys (keys are unique); broadcast reads, no
+    // stage barriers. Ranked entries go to tile[] / voff[] first, then back to skey/sslot.
+    wsync();
+    {
+      int32_t mykey[kHashCap / kWave], myslot[kHashCap / kWave], rank[kHashCap / kWave];
+#pragma unroll
+      for (int t = 0; t < kHashCap / kWave; ++t) {
+        const int j = t * kWave + lane;
+        mykey[t] = j < D ? h.skey[j] : INT32_MAX;
+        myslot[t] = j < D ? h.sslot[j] : -1;
+        rank[t] = 0;
+      }
+      for (int i = 0; i < D; ++i) {
+        const int32_t ki = h.skey[i];
+#pragma unroll
+        for (int t = 0; t < kHashCap / kWave; ++t) rank[t] += ki < mykey[t] ? 1 : 0;
+      }
+      wsync();
+#pragma unroll
+      for (int t = 0; t < kHashCap / kWave; ++t) {
+        if (t * kWave + lane < D) {
+          h.skey[rank[t]] = mykey[t];
+          h.sslot[rank[t]] = myslot[t];
+        }
+      }
+    }
+    wsync();
+
+    // ---- kept blocks: tile indices and value offsets by prefix sums over the sorted list
+    int32_t tile_base = 0, val_base = 0, d_base = 0;
+    if (FILL) {
+      tile_base = o.nt[s];
+      val_base = o.nv[s];
+      d_base = o.nd[s];
+    }
+    int32_t kcar = 0, vcar = 0;
+    for (int j0 = 0; j0 < D; j0 += kWave) {
+      const int j = j0 + lane;
+      const bool valid = j < D;
+      const int slot = valid ? h.sslot[j] : 0;
+      const uint64_t code = valid ? (((uint64_t)h.hi[slot] << 32) | h.lo[slot]) : 0ull;
+      const int32_t cnt = __popcll(code);
+      const bool kept = valid && cnt >= a.min_nnz;
+      const uint64_t km = __ballot(kept);
+      const int32_t krank = kcar + __popcll(km & below_lane);
+      int32_t vin = kept ? cnt : 0;  // inclusive scan of kept counts
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t y = __shfl_up(vin, d, kWave);
+        if (lane >= d) vin += y;
+      }
+      const int32_t vex = vcar + vin - (kept ? cnt : 0);
+      if (valid) {
+        const int32_t t = kept ? (FILL ? tile_base + krank : 1) : -1;
+        h.tile[slot] = t;
+        h.voff[slot] = val_base + vex;
+        if (FILL) {
+          const int32_t key = h.skey[j];
+          if (kept) {
+            o.codes[t] = code;
+            o.coords[2 * (int64_t)t] = (int32_t)r0;
+            o.coords[2 * (int64_t)t + 1] = key * bc;
+            o.offsets[t] = val_base + vex;
+          }
+          if (o.blk_bc != nullptr) {
+            o.blk_bc[d_base + j] = key;
+            o.blk_rmin[d_base + j] = (int8_t)h.rmin[slot];
+            o.blk_tile[d_base + j] = kept ? t : -1;
+          }
+        }
+      }
+      kcar += __popcll(km);
+      vcar += __shfl(vin, kWave - 1, kWave);
+    }
+    }  // FILL
+    wsync();
+
+    // ---- entries: kept values into their tile, rejected ones to the remainder row
+    for (int32_t c = E0; c < E1; c += kWave) {
+      const int32_t e = c + lane;
+      const bool valid = e < E1;
+      int r = 0, slot = 0, bit = 0;
+      int32_t col = 0;
+      if (valid) {
+        col = a.indices[e];
+        r = row_at(e);
+        const int32_t b = col / bc;
+        bit = r * bc + (col - b * bc);
+        slot = hash_slot(b);
+        while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
+      }
+      const bool rej = valid && h.tile[slot] < 0;
+      const uint64_t rm = __ballot(rej);
+      // lanes of this row in this chunk: [ls, ...]
+      const int ls = valid ? max(0, h.rows[r] - c) : 0;
+      const uint64_t from_ls = ls == 0 ? ~0ull : ~(~0ull >> (64 - ls));
+      const int32_t before = __popcll(rm & below_lane & from_ls);
+      if (FILL && valid) {
+        if (rej) {
+          const int32_t pos = o.nr[r0 + r] + h.rcnt[r] + before;
+          o.rem_col[pos] = col;
+          ((V*)o.rem_val)[pos] = vals[e];
+        } else {
+          const uint64_t code = ((uint64_t)h.hi[slot] << 32) | h.lo[slot];
+          const uint64_t below = bit == 0 ? 0ull : (~0ull >> (64 - bit));
+          ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = vals[e];
+        }
+      }
+      // the last lane of each row in this chunk advances that row's count
+      const bool last = valid && (lane == kWave - 1 || e + 1 == h.rows[r + 1]);
+      const int32_t in_row = __popcll(rm & (below_lane | (1ull << lane)) & from_ls);
+      wsync();
+      if (last) h.rcnt[r] += in_row;
+      wsync();
+    }
+    if (!FILL && lane < br && r0 + lane < a.rows) o.nr[r0 + lane] = h.rcnt[lane];
+    wsync();
   }
 }
 
@@ -279,6 +562,7 @@ int excl_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, hipStrea
 // ---- workspace layout ----
 struct BuildWs {
   int32_t *nt, *nv, *nd, *tmp;
+  uint8_t* big;
   int64_t bytes;
 };
 static int64_t al(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -296,6 +580,7 @@ static BuildWs carve_build(char* base, int64_t rows, int32_t br) {
   w.nv = take((ns + 1) * 4);
   w.nd = take((ns + 1) * 4);
   w.tmp = take(tmpn * 4);
+  w.big = (uint8_t*)take(ns);
   w.bytes = off;
   return w;
 }
@@ -308,6 +593,13 @@ static int group_lanes(int br) {
 
 template <bool FILL, typename V>
 static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
+  if (o.big != nullptr) {
+    const int64_t cap = (int64_t)device_info().num_cus * 16;  // waves
+    const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
+    hipLaunchKernelGGL((k_build_hash<FILL, V>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
+                       st, a, o);
+    GL_LAUNCH_CHECK("k_build_hash");
+  }
   const int GLn = group_lanes(a.br);
   const int spw = kWave / GLn;
   const int64_t waves = (a.num_stripes + spw - 1) / spw;
@@ -420,6 +712,8 @@ int gl_pbr_build_count(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   o.nv = w.nv;
   o.nr = rem_indptr;
   o.nd = w.nd;
+  o.big = w.big;
+  if (ns > 0) GL_HIP(hipMemsetAsync(w.big, 0, ns, st));
   if (ns > 0 && (rc = launch_build<false, uint32_t>(a, o, st))) return rc;
   if ((rc = excl_scan(w.nt, w.nt, ns, w.tmp, st))) return rc;
   if ((rc = excl_scan(w.nv, w.nv, ns, w.tmp, st))) return rc;
@@ -471,6 +765,7 @@ int gl_pbr_build_fill(int64_t rows, int64_t cols, int32_t block_rows, int32_t bl
   o.blk_bc = blk_bc;
   o.blk_rmin = blk_rmin;
   o.blk_tile = blk_tile;
+  o.big = w.big;  // as the count pass left it on this workspace
   if (ns > 0) {
     rc = value_bytes == 4 ? launch_build<true, uint32_t>(a, o, st)
                           : launch_build<true, uint64_t>(a, o, st);

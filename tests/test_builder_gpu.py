@@ -36,7 +36,8 @@ def _both(A, br, bc, mn, order):
 
 @pytest.mark.parametrize("order", ["reference", "canonical"])
 @pytest.mark.parametrize("N,density", [(64, 0.05), (128, 0.15), (1024, 0.05), (1000, 0.01)])
-@pytest.mark.parametrize("block", [(2, 2), (4, 4), (8, 8), (2, 4), (3, 5), (1, 64), (8, 1)])
+@pytest.mark.parametrize("block", [(2, 2), (4, 4), (8, 8), (2, 4), (3, 5), (1, 64), (8, 1),
+                                   (64, 1)])
 @pytest.mark.parametrize("min_nnz", [0, 1, 2, 4])
 def test_builder_random(order, N, density, block, min_nnz, device):
     A = random_csr(N, density, seed=N + min_nnz)
@@ -80,6 +81,26 @@ def test_builder_hub_stripe(device):
     key = np.unique(rows.astype(np.int64) * n + cols)
     A = sp.csr_matrix((np.arange(key.size, dtype=np.float32), (key // n, key % n)), shape=(n, n))
     _both(A, 8, 8, 2, "reference")
+
+
+@pytest.mark.parametrize("k", [255, 256, 257, 600])
+def test_builder_stripe_table_capacity(k, device):
+    """Stripes with k distinct blocks around the wave-per-stripe table size (256 blocks):
+    the ones that do not fit are built by the merge path, with identical results."""
+    n = 8 * 700
+    rng = np.random.default_rng(k)
+    rows = [np.full(k, 5), np.full(k // 2, 9)]            # stripe 0: k blocks, stripe 1: k/2
+    cols = [np.arange(k) * 8 + 3, np.arange(k // 2) * 8 + 1]
+    rows.append(rng.integers(16, n, 20_000))              # ordinary stripes
+    cols.append(rng.integers(0, n, 20_000))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols) % n
+    key = np.unique(r.astype(np.int64) * n + c)
+    A = sp.csr_matrix((rng.random(key.size).astype(np.float32), (key // n, key % n)),
+                      shape=(n, n))
+    for mn in (1, 2):
+        _both(A, 8, 8, mn, "reference")
+        _both(A, 8, 8, mn, "canonical")
 
 
 def test_noncanonical_input_is_canonicalized(device):
