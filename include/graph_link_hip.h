@@ -221,11 +221,17 @@ int gl_ws_pair_features(int64_t N, int64_t K, float alpha, const int32_t* indeg,
 /* Features of the (in-bucket, out-bucket, hop class 3/4/more) table, gl_ws_table_size() rows. */
 int64_t gl_ws_table_size(void);
 int gl_ws_table_features(float* feat, void* stream);
+/* Table row of every node: nb[i] = in-bucket * gl_ws_num_buckets() + out-bucket (once per
+ * graph, from gl_ws_degrees). */
+int gl_ws_node_buckets(int64_t n, const int32_t* indeg, const int32_t* outdeg, int32_t* nb,
+                       void* stream);
 /* X0 from the table for pairs beyond hop 2; pairs within hop 2 appended to near (count in
- * *near_count on the device, list truncated at near_cap). */
+ * *near_count on the device, list truncated at near_cap). node_buckets from
+ * gl_ws_node_buckets. */
 int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
-                   const int32_t* outdeg, const float* const* v5, const float* table, float* x0,
-                   int64_t* near, int64_t near_cap, uint64_t* near_count, void* stream);
+                   const int32_t* outdeg, const float* const* v5, const int32_t* node_buckets,
+                   const float* table, float* x0, int64_t* near, int64_t near_cap,
+                   uint64_t* near_count, void* stream);
 /* x0[pairs[m]] = val[m]. */
 int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, void* stream);
 /* Scale every column of x [N][K] to sum 1 (sums: K floats of scratch). */

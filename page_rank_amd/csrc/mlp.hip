@@ -298,25 +298,48 @@ __global__ void k_table_features(float* __restrict__ feat) {
   f[11] = 1.f;
 }
 
-// X0 for pairs beyond hop 2 from the table; pairs within hop 2 appended to `near`.
-__global__ void k_assemble(WsView s, const float* __restrict__ table, float* __restrict__ x0,
+// Table row of every node: in-bucket * kBuckets + out-bucket (once per graph).
+__global__ void k_node_buckets(int64_t n, const int32_t* __restrict__ indeg,
+                               const int32_t* __restrict__ outdeg, int32_t* __restrict__ nb) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    nb[i] = deg_bucket(indeg[i]) * kBuckets + deg_bucket(outdeg[i]);
+}
+
+// X0 for pairs beyond hop 2 from the table; pairs within hop 2 appended to `near`. One wave
+// per row (lanes over the K columns, coalesced in the [N][K] layout).
+__global__ void k_assemble(WsView s, const int32_t* __restrict__ nb,
+                           const float* __restrict__ table, float* __restrict__ x0,
                            int64_t* __restrict__ near, int64_t cap,
                            unsigned long long* __restrict__ count) {
-  const int64_t total = s.N * s.K;
-  for (int64_t ix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ix < total;
-       ix += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = ix / s.K;
-    int hop = 5;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t i = w0; i < s.N; i += nw) {
+    const int64_t tb = (int64_t)nb[i] * 3;
+    for (int64_t k = lane; k < s.K; k += 64) {
+      const int64_t ix = i * s.K + k;
+      int hop = 5;
 #pragma unroll
-    for (int j = 4; j >= 0; --j)
-      if (s.v[j][ix] > 0.f) hop = j;
-    if (hop <= 2) {
-      const unsigned long long slot = atomicAdd(count, 1ull);
-      if ((int64_t)slot < cap) near[slot] = ix;
-      x0[ix] = 0.f;
-    } else {
-      const int bi = deg_bucket(s.indeg[i]), bo = deg_bucket(s.outdeg[i]);
-      x0[ix] = table[((int64_t)bi * kBuckets + bo) * 3 + (hop - 3)];
+      for (int j = 4; j >= 0; --j)
+        if (s.v[j][ix] > 0.f) hop = j;
+      // near pairs: one counter update per wave (the lowest near lane reserves the slots)
+      const bool is_near = hop <= 2;
+      const uint64_t m = __ballot(is_near);
+      unsigned long long base = 0;
+      if (m != 0ull) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader, 64);
+      }
+      if (is_near) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const unsigned long long slot = base + __popcll(m & below);
+        if ((int64_t)slot < cap) near[slot] = ix;
+        x0[ix] = 0.f;
+      } else {
+        x0[ix] = table[tb + (hop - 3)];
+      }
     }
   }
 }
@@ -327,22 +350,36 @@ __global__ void k_scatter_pairs(const int64_t* __restrict__ pairs, const float* 
   if (m < M) x0[pairs[m]] = val[m];
 }
 
-// column sums and normalization of X0
-__global__ void k_colsum(const float* __restrict__ x, int64_t N, int64_t K, float* __restrict__ sums) {
+// Column sums of x [N][K]: a wave walks rows, lane k keeps column k (+64j) in registers; one
+// atomic add per column and workgroup at the end.
+__global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ x, int64_t N, int64_t K,
+                                                float* __restrict__ sums) {
   __shared__ float acc[1024];
   for (int k = threadIdx.x; k < K; k += blockDim.x) acc[k] = 0.f;
   __syncthreads();
-  for (int64_t ix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ix < N * K;
-       ix += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&acc[ix % K], x[ix]);
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k0 = 0; k0 < K; k0 += 64) {
+    const int64_t k = k0 + lane;
+    float a = 0.f;
+    if (k < K)
+      for (int64_t i = w0; i < N; i += nw) a += x[i * K + k];
+    if (k < K) atomicAdd(&acc[k], a);
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < K; k += blockDim.x) atomicAdd(&sums[k], acc[k]);
 }
 __global__ void k_colscale(float* __restrict__ x, int64_t N, int64_t K, const float* __restrict__ sums) {
-  for (int64_t ix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ix < N * K;
-       ix += (int64_t)gridDim.x * blockDim.x) {
-    const float s = sums[ix % K];
-    x[ix] = s > 0.f ? x[ix] / s : 0.f;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k0 = 0; k0 < K; k0 += 64) {
+    const int64_t k = k0 + lane;
+    if (k >= K) continue;
+    const float sk = sums[k];
+    const float inv = sk > 0.f ? 1.f / sk : 0.f;
+    for (int64_t i = w0; i < N; i += nw) x[i * K + k] *= inv;
   }
 }
 
@@ -458,15 +495,26 @@ int gl_ws_table_features(float* feat, void* stream) {
   return GL_OK;
 }
 
+int gl_ws_node_buckets(int64_t n, const int32_t* indeg, const int32_t* outdeg, int32_t* nb,
+                       void* stream) {
+  GL_REQUIRE(n >= 0 && (n == 0 || (indeg && outdeg && nb)), "bad arguments");
+  if (n == 0) return GL_OK;
+  hipLaunchKernelGGL(k_node_buckets, dim3(nblocks(n)), dim3(256), 0, (hipStream_t)stream, n,
+                     indeg, outdeg, nb);
+  GL_LAUNCH_CHECK("k_node_buckets");
+  return GL_OK;
+}
+
 int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
-                   const int32_t* outdeg, const float* const* v5, const float* table, float* x0,
-                   int64_t* near, int64_t near_cap, uint64_t* near_count, void* stream) {
-  GL_REQUIRE(v5 && table && x0 && near && near_count, "null pointer");
+                   const int32_t* outdeg, const float* const* v5, const int32_t* node_buckets,
+                   const float* table, float* x0, int64_t* near, int64_t near_cap,
+                   uint64_t* near_count, void* stream) {
+  GL_REQUIRE(v5 && node_buckets && table && x0 && near && near_count, "null pointer");
   WsView s = make_view(N, K, alpha, indeg, outdeg, v5);
   hipStream_t st = (hipStream_t)stream;
   GL_HIP(hipMemsetAsync(near_count, 0, 8, st));
-  hipLaunchKernelGGL(k_assemble, dim3(nblocks(N * K, 256, 16384)), dim3(256), 0, st, s, table,
-                     x0, near, near_cap, (unsigned long long*)near_count);
+  hipLaunchKernelGGL(k_assemble, dim3(nblocks(N * 64, 256, 16384)), dim3(256), 0, st, s,
+                     node_buckets, table, x0, near, near_cap, (unsigned long long*)near_count);
   GL_LAUNCH_CHECK("k_assemble");
   return GL_OK;
 }
@@ -484,9 +532,9 @@ int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream) {
   GL_REQUIRE(x && sums && K <= 1024, "bad arguments");
   hipStream_t st = (hipStream_t)stream;
   GL_HIP(hipMemsetAsync(sums, 0, K * 4, st));
-  hipLaunchKernelGGL(k_colsum, dim3(nblocks(N * K, 256, 2048)), dim3(256), 0, st, x, N, K, sums);
+  hipLaunchKernelGGL(k_colsum, dim3(nblocks(N * 64, 256, 1024)), dim3(256), 0, st, x, N, K, sums);
   GL_LAUNCH_CHECK("k_colsum");
-  hipLaunchKernelGGL(k_colscale, dim3(nblocks(N * K, 256, 16384)), dim3(256), 0, st, x, N, K,
+  hipLaunchKernelGGL(k_colscale, dim3(nblocks(N * 64, 256, 16384)), dim3(256), 0, st, x, N, K,
                      sums);
   GL_LAUNCH_CHECK("k_colscale");
   return GL_OK;

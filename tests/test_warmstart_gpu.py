@@ -77,6 +77,8 @@ def test_features_match_oracle(device):
     outdeg = np.bincount(A.indices, minlength=A.shape[0])
     assert np.array_equal(wi.indeg.cpu().numpy(), indeg)
     assert np.array_equal(wi.outdeg.cpu().numpy(), outdeg)
+    nb_want = omlp.deg_bucket(indeg) * 193 + omlp.deg_bucket(outdeg)
+    assert np.array_equal(wi.nb.cpu().numpy(), nb_want.astype(np.int64))
     P = oppr.personalization_matrix(A.shape[0], sets)
     v = [P]
     for _ in range(4):
