@@ -2,7 +2,19 @@
 #pragma once
 
 #include "gl_common.hpp"
+#include "mfma_kernels.hpp"
 #include "stripe_kernels.hpp"
+
+#include <type_traits>
+
+// Matrix-core tile product for 32 / 64 columns of densely tiled 8x8 SpMMs (1 = on).
+// Measured on MI355X (1M x 1M, 8 tiles per stripe): 10 % slower at 32 columns (block part
+// 202 vs 181 us) and 4 % faster at 64 (494 vs 516 us). Pairing two 8-row stripes in one
+// 16x16x4 product leaves half of every MFMA multiplying zeros, so the matrix pipe does no
+// more useful work per cycle than the VALU FMAs it replaces. Off.
+#ifndef GL_MFMA
+#define GL_MFMA 0
+#endif
 
 namespace gl {
 
@@ -20,6 +32,16 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
   // tile batch: a whole stripe at once when 8x8 stripes hold about 8 tiles (measured on
   // MI355X: -16 % at 1 column, -3 % at 32; narrower batches win on sparser tilings)
   const bool wide = B == 8 && m.num_tiles >= 6 * m.num_stripes;
+  if constexpr (GL_MFMA && B == 8 && TILES && std::is_same<Epi, EpiStore>::value) {
+    if (wide && (kw == 32 || kw == 64)) {
+      if (kw == 32)
+        hipLaunchKernelGGL((k_stripe_mfma<32, REM, BUF>), grid, block, 0, st, m, d, e);
+      else
+        hipLaunchKernelGGL((k_stripe_mfma<64, REM, BUF>), grid, block, 0, st, m, d, e);
+      GL_LAUNCH_CHECK("k_stripe_mfma");
+      return GL_OK;
+    }
+  }
   switch (kw) {
 #define GL_KB(KW)                                                                             \
     case KW:                                                                                  \

@@ -54,6 +54,15 @@
 #ifndef GL_REM_FLAT_EB
 #define GL_REM_FLAT_EB 4
 #endif
+// Feature-lane remainder (row lanes): issue the X-row gathers of the next entry batch, and
+// the column indices of the next block of entries, before the FMAs of the current batch
+// (1), or load and consume one batch at a time (0). Measured on MI355X: 13 % slower for
+// SpMM at 32 columns, 21 % for the 32-column PPR step and 79 % for the 64-column one (the
+// second batch spills under the 128-VGPR cap); with 1- and 2-entry batches, 1-14 % slower.
+// So it is off.
+#ifndef GL_REM_PIPE
+#define GL_REM_PIPE 0
+#endif
 // Remainder entries whose X-row gathers are issued together (feature-lane kernel).
 #ifndef GL_REM_EB
 #define GL_REM_EB 4
@@ -531,6 +540,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   // per lane and registers decide the occupancy)
   constexpr int EBW = KW >= 64 ? GL_REM_EB64 : GL_REM_EB;
   constexpr int EB = L < EBW ? L : EBW;
+  static_assert(L % EB == 0, "remainder batches must not straddle a block of L entries");
   // flat remainder: LE lanes (4 columns each) per entry, NGR entry groups per wave
   constexpr int LE = KW / 4;
   constexpr int NGR = kWave / LE;
@@ -676,6 +686,16 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
       sval = ld.rval.f32(rs.j0 + qq, ok);
     }
+#if GL_REM_PIPE
+    // ... and of the second block of L entries
+    int ncol = 0;
+    float nval = 0.f;
+    if constexpr (REM) {
+      const bool ok = L + qq < rs.n;
+      ncol = (int)ld.rcol.u32(rs.j0 + L + qq, ok);
+      nval = ld.rval.f32(rs.j0 + L + qq, ok);
+    }
+#endif
 #endif
 
     float acc[B];
@@ -783,6 +803,48 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
 #pragma unroll
       for (int i = 0; i < FPL; ++i) racc2[i] = 0.f;
       const int nmax = wave_max(rs.n);
+#if GL_REM_PIPE
+      // Batches of EB entries (EB divides L, so a batch lies inside one block of L entries
+      // whose columns and values are spread over the row's L lanes). Two batch buffers:
+      // the gathers of batch e + EB are in flight while batch e is multiplied.
+      struct RemBatch {
+        float x[EB][FPL];
+        float a[EB];
+      };
+      auto issue = [&](RemBatch& b, int e) {
+        if (e % L == 0 && e > 0) {  // next block: its indices were requested a block ago
+          scol = ncol;
+          sval = nval;
+          const bool ok = e + L + qq < rs.n;
+          ncol = (int)ld.rcol.u32(rs.j0 + e + L + qq, ok);
+          nval = ld.rval.f32(rs.j0 + e + L + qq, ok);
+        }
+#pragma unroll
+        for (int i = 0; i < EB; ++i) {
+          const int c = __shfl(scol, gbase + (e % L) + i, kWave);
+          b.a[i] = __shfl(sval, gbase + (e % L) + i, kWave);
+          const bool ok = e + i < rs.n;
+          load_row<FPL>(ld.x, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, b.x[i]);
+        }
+      };
+      auto consume = [&](const RemBatch& b) {
+#pragma unroll
+        for (int i = 0; i < EB; ++i)
+#pragma unroll
+          for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(b.a[i], b.x[i][u], racc2[u]);
+      };
+      if (nmax > 0) {
+        RemBatch b0, b1;
+        issue(b0, 0);
+        for (int e = 0; e < nmax; e += 2 * EB) {
+          const bool more = e + EB < nmax;
+          if (more) issue(b1, e + EB);
+          consume(b0);
+          if (e + 2 * EB < nmax) issue(b0, e + 2 * EB);
+          if (more) consume(b1);
+        }
+      }
+#else
       for (int jb = 0; jb < nmax; jb += L) {
         if (jb > 0) {
           const bool ok = jb + qq < rs.n;
@@ -805,6 +867,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
             for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(a[i], xv[i][u], racc2[u]);
         }
       }
+#endif
       // transpose through LDS: (row rr, columns kq..) -> (column f, all rows)
       if (kq < KW) {
 #pragma unroll

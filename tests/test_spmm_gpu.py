@@ -174,3 +174,35 @@ def test_block_only_product_named_like_the_reference(device):
     graph_link.pbr_batched_matmul_triton_gpu(pbr, torch.from_numpy(X3).to(device), y3, 2, 12)
     for b in range(2):
         assert np.allclose(y3[b].cpu().numpy(), tiles_only.dot(X3[b]), atol=1e-4)
+
+
+@pytest.mark.parametrize("kw", [0, 16, 32, 64])
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_spmm_densely_tiled_8x8(kw, mode, device):
+    """8x8 tilings with about ten tiles per stripe (the 1M x 1M workload's shape, which takes
+    the whole-stripe tile batches and, at 32 and 64 columns, the matrix-core product): odd
+    stripe count, a partial last stripe, an empty stripe beside a full one, unequal tile
+    counts in adjacent stripes, a scattered remainder, strided X and Y, accumulate."""
+    n = 8 * 41 + 3
+    A = blocky_csr(n, 8, 12, 20, 64, scatter=900, seed=13, band=8).tolil()
+    A[16:24, :] = 0                       # stripe 2 empty, stripe 3 full
+    A[40:48, :] = 0
+    A[40:48, 0:8] = 1.0                   # stripe 5: one tile
+    A = A.tocsr()
+    A.eliminate_zeros()
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
+    ns = (n + 7) // 8
+    assert pbr.accounted_blocks() >= 6 * ns  # the dense-tiling launch
+    for K in (32, 64, 20, 97, 128):
+        X = random_x(n, K, seed=K)
+        xd = torch.from_numpy(X).to(device)
+        Y = graph_link.pbr_matmul(pbr, xd, mode=mode, kw=kw).cpu().numpy()
+        _check(A, X, Y)
+        # strided operands (row pitch not a multiple of 4) and accumulate
+        xs = torch.zeros((n, K + 3), device=device)
+        xs[:, :K] = xd
+        ys = torch.full((n, K + 5), 2.0, device=device)
+        graph_link.spmm_into(pbr, xs[:, :K], ys[:, :K], accumulate=True, mode=mode, kw=kw)
+        out = ys.cpu().numpy()
+        _check(A, X, out[:, :K] - 2.0)
+        assert np.all(out[:, K:] == 2.0)
