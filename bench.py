@@ -15,6 +15,7 @@ baseline (the CPU restatement of the reference's pbr_batched_matmul_cpu, one thr
 row sample of the same matrix).
 
     python bench.py [--steps 20] [--warmup 5] [--k 32] [--widths 1,8,32] [--no-cpu]
+    python bench.py --full-scale      # the 16M-node / 256M-edge PPR run only
 """
 from __future__ import annotations
 
@@ -49,13 +50,24 @@ def parse():
     ap.add_argument("--ppr-nodes", type=int, default=4_000_000)
     ap.add_argument("--ppr-edges", type=int, default=64_000_000)
     ap.add_argument("--ppr-sets", type=int, default=64)
+    ap.add_argument("--ppr-set-max", type=int, default=100,
+                    help="seed-set sizes are uniform in 1..this")
     ap.add_argument("--ppr-check-cols", type=int, default=2,
                     help="columns checked against the FP64 CPU power iteration (and timed "
                          "as the CPU baseline of the PPR leg)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a kernel of "
                          "known traffic that scales the profiler's FETCH_SIZE / WRITE_SIZE)")
-    return ap.parse_args()
+    ap.add_argument("--full-scale", action="store_true",
+                    help="only the full-scale PPR run: 16M nodes, 256M edges, 16,000 "
+                         "communities, 128 seed sets of U{1..200} nodes, learned warm start "
+                         "beside plain, FP64 errors on 8 sampled columns")
+    args = ap.parse_args()
+    if args.full_scale:
+        args.ppr_only = True
+        args.ppr_nodes, args.ppr_edges = 16_000_000, 256_000_000
+        args.ppr_sets, args.ppr_set_max, args.ppr_check_cols = 128, 200, 8
+    return args
 
 
 def log(*a):
@@ -150,10 +162,12 @@ def ppr_matrix_bytes(T, V, R, N):
 
 
 def ppr_leg(args, dev):
-    """Batched multi-seed PPR on the block-clustered power-law graph (4M nodes, 64M edges,
-    4,000 communities, 80 % intra, out-degree Pareto(2.1, min 4)), 64 seed sets of U{1..100}
-    nodes, alpha 0.85, per-column L1 tol 1e-6, hipGraph-captured loop. Errors against the
-    FP64 CPU power iteration on sampled columns, which is also the CPU baseline."""
+    """Batched multi-seed PPR on the block-clustered power-law graph (by default 4M nodes,
+    64M edges, 4,000 communities, 80 % intra, out-degree Pareto(2.1, min 4)), 64 seed sets
+    of U{1..100} nodes, alpha 0.85, per-column L1 tol 1e-6, hipGraph-captured loop; with
+    --full-scale 16M nodes, 256M edges, 16,000 communities and 128 sets of U{1..200}.
+    Errors against the FP64 CPU power iteration on sampled columns, which is also the CPU
+    baseline."""
     import numpy as np
     import torch
     import page_rank_amd as gl
@@ -161,8 +175,10 @@ def ppr_leg(args, dev):
     n, m, K = args.ppr_nodes, args.ppr_edges, args.ppr_sets
     log(f"generating the {n} node / {m} edge clustered graph ...")
     src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000), seed=3)
+    log("  edges drawn; building the column-stochastic CSR ...")
     A, dang = graphs.transition_matrix(src, dst, n)
     del src, dst
+    log("  CSR built; building the PBR on the GPU ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pbr = gl.csr_to_pbr(A, 8, 8, 4, order="canonical")
@@ -171,7 +187,8 @@ def ppr_leg(args, dev):
     T, V, R = pbr.accounted_blocks(), pbr.compressed_nnz(), pbr.remainder_nnz()
     log(f"  nnz={A.nnz} T={T} V={V} R={R} dangling={int(dang.sum())}, built in "
         f"{t_build*1e3:.0f} ms (incl. upload)")
-    sets = graphs.seed_sets(n, K, 1, 100, seed=4)
+    smax = args.ppr_set_max
+    sets = graphs.seed_sets(n, K, 1, smax, seed=4)
     dmode = "uniform" if dang.any() else "none"
     kw = dict(alpha=0.85, tol=1e-6, max_iter=300, dangling=dang if dang.any() else None,
               dangling_mode=dmode, retire="per_column", return_history=False)
@@ -191,7 +208,7 @@ def ppr_leg(args, dev):
         f"mean {res.iterations.mean():.1f}, {gbs:.0f} GB/s algorithmic")
     out = {"workload": f"ppr-{n // 1_000_000}M-{m // 1_000_000}M-clustered-{K}sets",
            "nodes": n, "edges_nnz": int(A.nnz), "tiles": T, "tile_nnz": V, "remainder_nnz": R,
-           "seed_sets": K, "seed_set_size": "U{1..100}", "alpha": 0.85, "tol": 1e-6,
+           "seed_sets": K, "seed_set_size": f"U{{1..{smax}}}", "alpha": 0.85, "tol": 1e-6,
            "dangling_mode": dmode, "ppr_per_s": round(K / t, 3), "ms": round(t * 1e3, 3),
            "iterations_max": res.total_iterations,
            "iterations_mean": round(float(res.iterations.mean()), 2),
@@ -240,14 +257,24 @@ def ppr_leg(args, dev):
         from oracle import ppr as oppr
         cols = np.random.default_rng(5).choice(K, min(K, args.ppr_check_cols), replace=False)
         P = oppr.personalization_matrix(n, [sets[c] for c in cols])
+
+        def progress(t, err):
+            if t % 10 == 0:
+                log(f"    FP64 iteration {t}: max residual {err.max():.2e}")
+        log(f"  FP64 CPU power iteration on {len(cols)} columns ...")
         t0 = time.perf_counter()
         X64, it64, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-6, max_iter=300,
                                      dangling=dang if dang.any() else None,
-                                     dangling_mode=dmode, retire="per_column")
+                                     dangling_mode=dmode, retire="per_column",
+                                     progress=progress)
         t_cpu = time.perf_counter() - t0
-        Xref, _, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-13, max_iter=600,
+        # reference: the same iteration continued to tol 1e-11 (error below 6e-11 per
+        # column, five orders under the errors reported)
+        log(f"  ... {t_cpu:.1f} s; continuing to tol 1e-11 for the reference ...")
+        Xref, _, _, _ = oppr.power(A, P, alpha=0.85, tol=1e-11, max_iter=600,
                                    dangling=dang if dang.any() else None,
-                                   dangling_mode=dmode, retire="per_column", x0=X64)
+                                   dangling_mode=dmode, retire="per_column", x0=X64,
+                                   progress=progress)
         sel = torch.as_tensor(cols, device=res.X.device)
         X = res.X[:, sel].cpu().numpy().astype(np.float64)
         l1 = np.abs(X - Xref).sum(0)

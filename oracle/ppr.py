@@ -60,9 +60,10 @@ def reference_loop(A, sources, alpha=0.85, max_iterations=100, tolerance=1e-6,
 
 
 def power(A, P, alpha=0.85, tol=1e-6, max_iter=100, dangling=None, dangling_mode="none",
-          retire="together", x0=None):
+          retire="together", x0=None, progress=None):
     """FP64 batched PPR: X_{t+1} = alpha*A X_t + alpha*d_k*w + (1-alpha)*P, d_k = sum of
     X_t over dangling rows, w = 1/N (uniform) or P[:,k] (personalization).
+    `progress(t, err)` (optional) is called after every iteration with its residuals.
     Returns (X, iterations[K], converged[K], history[T][K])."""
     A = sp.csr_matrix(A, dtype=np.float64)
     n = A.shape[0]
@@ -87,6 +88,8 @@ def power(A, P, alpha=0.85, tol=1e-6, max_iter=100, dangling=None, dangling_mode
         err = np.abs(Xn - X).sum(axis=0)
         err[~active] = 0.0
         hist.append(err.copy())
+        if progress is not None:
+            progress(t + 1, err)
         X[:, active] = Xn[:, active]
         if retire == "together":
             done = np.full(K, err[active].max() < tol) & active

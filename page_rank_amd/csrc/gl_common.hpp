@@ -74,6 +74,13 @@ __device__ __forceinline__ Sched xcd_sched(int64_t n) {
   return s;
 }
 
+// LDS written by some lanes of a wave, then read by other lanes of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // row_newbcast:n (gfx90a+ DPP): every lane of a 16-lane row reads lane n of that row.
 template <int N>
 __device__ __forceinline__ float row_bcast(float v) {

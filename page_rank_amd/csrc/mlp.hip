@@ -307,41 +307,54 @@ __global__ void k_node_buckets(int64_t n, const int32_t* __restrict__ indeg,
 }
 
 // X0 for pairs beyond hop 2 from the table; pairs within hop 2 appended to `near`. One wave
-// per row (lanes over the K columns, coalesced in the [N][K] layout).
-__global__ void k_assemble(WsView s, const int32_t* __restrict__ nb,
-                           const float* __restrict__ table, float* __restrict__ x0,
-                           int64_t* __restrict__ near, int64_t cap,
-                           unsigned long long* __restrict__ count) {
+// per row (lanes over the K columns, coalesced in the [N][K] layout). Each wave collects its
+// near pairs in an LDS buffer and reserves their slots with one counter update per flush:
+// one update per row instead would serialize millions of atomics on the one counter.
+constexpr int kNearBuf = 512;  // pairs per wave buffer (>= 64: a row chunk always fits)
+__global__ void __launch_bounds__(256) k_assemble(WsView s, const int32_t* __restrict__ nb,
+                                                  const float* __restrict__ table,
+                                                  float* __restrict__ x0,
+                                                  int64_t* __restrict__ near, int64_t cap,
+                                                  unsigned long long* __restrict__ count) {
+  __shared__ int64_t buf_all[4][kNearBuf];
   const int lane = threadIdx.x & 63;
+  int64_t* __restrict__ buf = buf_all[threadIdx.x >> 6];
   const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int nbuf = 0;  // wave-uniform
+  auto flush = [&]() {
+    wave_lds_sync();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, (unsigned long long)nbuf);
+    base = __shfl(base, 0, 64);
+    for (int j = lane; j < nbuf; j += 64)
+      if ((int64_t)(base + j) < cap) near[base + j] = buf[j];
+    wave_lds_sync();
+    nbuf = 0;
+  };
   for (int64_t i = w0; i < s.N; i += nw) {
     const int64_t tb = (int64_t)nb[i] * 3;
-    for (int64_t k = lane; k < s.K; k += 64) {
+    for (int64_t k0 = 0; k0 < s.K; k0 += 64) {
+      const int64_t k = k0 + lane;
+      const bool ok = k < s.K;
       const int64_t ix = i * s.K + k;
       int hop = 5;
+      if (ok) {
 #pragma unroll
-      for (int j = 4; j >= 0; --j)
-        if (s.v[j][ix] > 0.f) hop = j;
-      // near pairs: one counter update per wave (the lowest near lane reserves the slots)
-      const bool is_near = hop <= 2;
+        for (int j = 4; j >= 0; --j)
+          if (s.v[j][ix] > 0.f) hop = j;
+      }
+      const bool is_near = ok && hop <= 2;
       const uint64_t m = __ballot(is_near);
-      unsigned long long base = 0;
-      if (m != 0ull) {
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader, 64);
-      }
-      if (is_near) {
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const unsigned long long slot = base + __popcll(m & below);
-        if ((int64_t)slot < cap) near[slot] = ix;
-        x0[ix] = 0.f;
-      } else {
-        x0[ix] = table[tb + (hop - 3)];
-      }
+      const int c = __popcll(m);
+      if (nbuf + c > kNearBuf) flush();
+      if (is_near) buf[nbuf + __popcll(m & below)] = ix;
+      nbuf += c;
+      if (ok) x0[ix] = is_near ? 0.f : table[tb + (hop - 3)];
     }
   }
+  if (nbuf > 0) flush();
 }
 
 __global__ void k_scatter_pairs(const int64_t* __restrict__ pairs, const float* __restrict__ val,

@@ -416,12 +416,6 @@ __device__ __forceinline__ float sum8(float v) {
   return v + dpp<0x141>(v);
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <bool BUF>
 __device__ __forceinline__ PbrLd<BUF>::PbrLd(const PbrView& m, const RhsView& d)
     : codes(m.codes, m.num_tiles * 8),

@@ -162,3 +162,27 @@ def test_errors(device):
     cpu = pbr.to("cpu")
     with pytest.raises(RuntimeError):
         graph_link.run_personalized_pagerank(cpu, torch.tensor([0], dtype=torch.int32))
+
+
+@pytest.mark.parametrize("K", [65, 130])
+def test_more_seed_sets_than_one_launch_width(K, device):
+    """More than 64 seed-set columns: several column chunks per iteration, per-column
+    retirement across chunks, dangling mass; FP64 oracle with the same stopping rule."""
+    n = 600
+    adj = _sbm(n, 100, 0.08, 0.004, seed=2).tolil()
+    adj[11, :] = 0
+    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    A32 = sp.csr_matrix(A, dtype=np.float32)
+    rng = np.random.default_rng(K)
+    seed_sets = [rng.choice(n, size=int(rng.integers(1, 30)), replace=False) for _ in range(K)]
+    pbr = graph_link.csr_to_pbr(A32, 8, 8, 2).to(device)
+    res = graph_link.run_ppr(pbr, seed_sets, alpha=0.85, tol=1e-6, max_iter=200,
+                             dangling=dangling, dangling_mode="uniform", retire="per_column",
+                             graph_iters=4)
+    P = oppr.personalization_matrix(n, seed_sets)
+    Xo, it_o, conv_o, _ = oppr.power(A32, P, 0.85, 1e-6, 200, dangling, "uniform", "per_column")
+    X = res.X.cpu().numpy()
+    assert X.shape == (n, K) and res.converged.all() and conv_o.all()
+    assert np.all(np.abs(res.iterations - it_o) <= 1)
+    assert np.abs(X - Xo).sum(axis=0).max() < 2e-6
+    assert np.allclose(X.sum(axis=0), 1.0, atol=1e-5)

@@ -127,3 +127,37 @@ def test_trained_network_saves_iterations(device):
     assert ev.max_l1_diff < 1e-5
     assert ev.start_l1_warm.mean() < ev.start_l1_plain.mean()
     assert ev.warm_iters.mean() < ev.plain_iters.mean()
+
+
+def test_assembly_lists_every_near_pair_once(device):
+    """1M nodes, 130 seed sets of 20,000 nodes: most (node, set) pairs lie within two hops,
+    so every wave of the assembly kernel flushes its near-pair buffer several times. The
+    near list must hold each pair with hop <= 2 exactly once (a lost pair would keep x0 = 0,
+    a repeated one would inflate the count), and every entry of X0 must be the network's
+    output for that pair's features divided by its column's sum."""
+    n, K = 1_000_000, 130
+    rng = np.random.default_rng(7)
+    src = np.repeat(np.arange(n), 6)
+    dst = rng.integers(0, n, src.size)
+    A, _ = graphs.transition_matrix(src, dst, n)
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
+    sets = [rng.choice(n, 20_000, replace=False) for _ in range(K)]
+    p = ws.init_params(3, device)
+    wi = ws.inputs(pbr, sets, 0.85)
+    want = int(((wi.v[0] > 0) | (wi.v[1] > 0) | (wi.v[2] > 0)).sum().item())
+    x0, n_near = ws.predict(p, wi)
+    assert n_near == want and n_near > n * K // 4
+    assert bool((x0 > 0).all())
+    # a sample of pairs through the per-pair path: ratio to X0 is 1 / column sum
+    pairs = torch.as_tensor(np.sort(rng.choice(n * K, 200_000, replace=False)), device=device)
+    ref = ws.forward(p, ws.pair_features(wi, pairs))
+    got = x0.view(-1)[pairs]
+    ratio = (got / ref).cpu().numpy()
+    cols = (pairs % K).cpu().numpy()
+    for k in range(K):
+        r = ratio[cols == k]
+        assert np.allclose(r, r[0], rtol=2e-5), k
+    assert torch.allclose(x0.sum(0), torch.ones(K, device=device), atol=1e-4)
+    # a near list smaller than the pair count is refused and predict retries with the count
+    x1, n1 = ws.predict(p, wi, near_cap=1 << 16)
+    assert n1 == want and torch.allclose(x1, x0, rtol=1e-5, atol=0)
