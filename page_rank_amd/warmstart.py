@@ -318,30 +318,39 @@ class WarmStartEval:
 
 
 def evaluate(params, pbr, seed_sets, dangling=None, dangling_mode="none", *, alpha=0.85,
-             tol=1e-6, max_iter=400, x_star=None):
+             tol=1e-6, max_iter=400, x_star=None, reps=3):
+    """Plain and warm-started runs to the same tol; times are medians over `reps` runs, the
+    warm time including features and network."""
     from .ppr import run_ppr
     dev = get_pbr_gpu_meta(pbr)["codes"].device
     kw = dict(alpha=alpha, tol=tol, max_iter=max_iter, dangling=dangling,
               dangling_mode=dangling_mode, return_history=False)
     run_ppr(pbr, seed_sets, **kw)                       # warm up graph capture / caches
     graph = graph_features(pbr)       # per graph, like the matrix itself
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    plain = run_ppr(pbr, seed_sets, **kw)
-    torch.cuda.synchronize(dev)
-    t1 = time.perf_counter()
-    wi = inputs(pbr, seed_sets, alpha, graph=graph)
-    x0, _ = predict(params, wi)
-    torch.cuda.synchronize(dev)
-    t2 = time.perf_counter()
-    warm = run_ppr(pbr, seed_sets, x0=x0, **kw)
-    torch.cuda.synchronize(dev)
-    t3 = time.perf_counter()
+    t_plain, t_warm, t_pred = [], [], []
+    for _ in range(max(1, reps)):
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        plain = run_ppr(pbr, seed_sets, **kw)
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        wi = inputs(pbr, seed_sets, alpha, graph=graph)
+        x0, _ = predict(params, wi)
+        del wi
+        torch.cuda.synchronize(dev)
+        t2 = time.perf_counter()
+        warm = run_ppr(pbr, seed_sets, x0=x0, **kw)
+        torch.cuda.synchronize(dev)
+        t3 = time.perf_counter()
+        t_plain.append(t1 - t0)
+        t_warm.append(t3 - t1)
+        t_pred.append(t2 - t1)
     if x_star is None:
         x_star = run_ppr(pbr, seed_sets, **{**kw, "tol": min(tol, 1e-7)}).X
     P = seed_matrix(pbr.rows, seed_sets, dev)
     return WarmStartEval(
-        plain.iterations, warm.iterations, (t1 - t0) * 1e3, (t3 - t1) * 1e3, (t2 - t1) * 1e3,
+        plain.iterations, warm.iterations, float(np.median(t_plain)) * 1e3,
+        float(np.median(t_warm)) * 1e3, float(np.median(t_pred)) * 1e3,
         (P - x_star).abs().sum(0).cpu().numpy(), (x0 - x_star).abs().sum(0).cpu().numpy(),
         float((plain.X - warm.X).abs().sum(0).max().item()))
 
@@ -364,6 +373,10 @@ def main(argv=None):
 
         python -m page_rank_amd.warmstart --graphs 20 --nodes 1000000 --edges 16000000 \
             --sets 100 --out page_rank_amd/data/warmstart_clustered.npz
+
+    or evaluate saved weights only:
+
+        python -m page_rank_amd.warmstart --load page_rank_amd/data/warmstart_clustered.npz
     """
     import argparse
     import json
@@ -377,7 +390,12 @@ def main(argv=None):
     ap.add_argument("--near-samples", type=int, default=1 << 18)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--load", default=None, help="evaluate these weights; no training")
     a = ap.parse_args(argv)
+    if a.load:
+        params = load(a.load, torch.device("cuda", torch.cuda.current_device()))
+        _evaluate_heldout(params, a, train_seconds=None, losses=None)
+        return
     t0 = time.perf_counter()
     feats, targets, weights = [], [], []
     dev = None
@@ -402,6 +420,11 @@ def main(argv=None):
     del feats, targets, weights
     if a.out:
         save(params, a.out)
+    _evaluate_heldout(params, a, round(log.seconds, 2), log.losses)
+
+
+def _evaluate_heldout(params, a, train_seconds, losses):
+    import json
     evals = []
     for h in range(a.heldout):
         pbr, sets, dang, dmode = clustered_sample(a.nodes, a.edges, a.sets, seed=5000 + h)
@@ -414,9 +437,9 @@ def main(argv=None):
                       "start_l1_warm": float(ev.start_l1_warm.mean()),
                       "max_l1_diff": ev.max_l1_diff})
         print(json.dumps(evals[-1]), flush=True)
-    print(json.dumps({"graphs": a.graphs, "nodes": a.nodes, "edges": a.edges, "sets": a.sets,
-                      "train_seconds": round(log.seconds, 2), "losses": log.losses,
-                      "heldout": evals}), flush=True)
+    print(json.dumps({"graphs": a.graphs if losses is not None else 0, "nodes": a.nodes,
+                      "edges": a.edges, "sets": a.sets, "train_seconds": train_seconds,
+                      "losses": losses, "heldout": evals}), flush=True)
 
 
 if __name__ == "__main__":
