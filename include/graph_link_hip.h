@@ -277,6 +277,35 @@ int gl_topk_columns_f32(const float* x, int64_t N, int64_t K, int64_t ld, int64_
                         float* values, int64_t* rows, void* workspace, int64_t workspace_bytes,
                         void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * FP64 variants (float64 PBR matrices), a compatibility path beside the FP32 kernels.
+ * ---------------------------------------------------------------------------------- */
+/* Y = A X (or Y += A X) for a float64 PBR and float64 X/Y, row-major [rows][K]; the
+ * block tiles and the remainder in one launch. Replaces pbr_spmm_cuda_i32_f64
+ * (Backend/graph_link/bindings/bindings_kernels.cu:13-83, bound at :144) with its
+ * launchers (Backend/graph_link/kernels/pbr_kernel.cu:336-396, :460-478). */
+int gl_pbr_spmm_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                        int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                        const int32_t* offsets, const double* data, const int32_t* stripe_ptr,
+                        const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                        const double* rem_vals, const double* X, int64_t ldx, int64_t K,
+                        double* Y, int64_t ldy, int32_t accumulate, void* stream);
+
+/* X [N][K] = one-hot columns, X[i][k] = (i == source_nodes[k]); the whole of X is written.
+ * Replaces init_ppr_cuda_double (Backend/graph_link/kernels/ppr_kernel.cu:199-216, bound at
+ * bindings_kernels.cu:149), which also needs X zeroed beforehand. */
+int gl_ppr_init_f64(double* X, const int32_t* source_nodes, int64_t N, int64_t K,
+                    void* stream);
+
+/* X <- alpha Y + (1 - alpha) e_src in place; errors[k] = L1 norm of column k's update.
+ * Replaces ppr_update_normalized_cuda_double (ppr_kernel.cu:130-187, bound at
+ * bindings_kernels.cu:161). The residual is summed in a fixed order (the reference uses
+ * atomics). The workspace holds per-block partial sums. */
+int64_t gl_ppr_update_f64_workspace_bytes(int64_t N, int64_t K);
+int gl_ppr_update_normalized_f64(const double* Y, double* X, const int32_t* source_nodes,
+                                 double alpha, int64_t N, int64_t K, double* errors,
+                                 void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,13 @@ _PROTOS = {
     "gl_topk_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "gl_topk_columns_f32": (ctypes.c_int, [c_p, c_i64, c_i64, c_i64, c_i64, c_p, c_p, c_p, c_i64,
                                            c_p]),
+    "gl_pbr_spmm_i32_f64": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_p, c_p, c_p,
+                                           c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_i64, c_p,
+                                           c_i64, c_i32, c_p]),
+    "gl_ppr_init_f64": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p]),
+    "gl_ppr_update_f64_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "gl_ppr_update_normalized_f64": (ctypes.c_int, [c_p, c_p, c_p, ctypes.c_double, c_i64,
+                                                    c_i64, c_p, c_p, c_i64, c_p]),
 }
 
 _lib = None

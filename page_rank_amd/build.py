@@ -19,7 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libgraphlink_hip.so")
 ARCH = os.environ.get("GL_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.hip", "ppr.hip", "builder.hip", "mlp.hip", "csr_ops.hip", "select.hip"]
+SOURCES = ["capi.hip", "ppr.hip", "builder.hip", "mlp.hip", "csr_ops.hip", "select.hip",
+           "f64.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -21,15 +21,18 @@ def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mo
     kw: RHS columns per launch (0 = library rule, "auto" = measured by the autotuner, which
     then also picks the mode)."""
     meta = _meta_or_raise(pbr_mat)
-    if x.dtype != torch.float32 or y.dtype != torch.float32:
+    vdt = meta["data"].dtype
+    if vdt not in (torch.float32, torch.float64) or x.dtype != vdt or y.dtype != vdt:
         raise TypeError(f"Unsupported dtype combination: index={meta['coords'].dtype}, "
-                        f"data={x.dtype}")
+                        f"data={vdt}, x={x.dtype}, y={y.dtype}")
     if x.dim() != 2 or x.shape[0] != pbr_mat.cols or y.shape != (pbr_mat.rows, x.shape[1]):
         raise ValueError(f"shape mismatch: A is {pbr_mat.rows}x{pbr_mat.cols}, x {tuple(x.shape)}"
                          f", y {tuple(y.shape)}")
     if x.stride(1) != 1 or y.stride(1) != 1:
         raise ValueError("x and y must have unit stride along K")
     K = x.shape[1]
+    if vdt == torch.float64:
+        return _spmm_into_f64(pbr_mat, meta, x, y, accumulate)
     if kw == "auto":
         from .autotune import tune_spmm
         t = tune_spmm(pbr_mat, K)
@@ -49,12 +52,29 @@ def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mo
     return y
 
 
+def _spmm_into_f64(pbr_mat, meta, x, y, accumulate):
+    """FP64 product (bindings_kernels.cu:144, pbr_spmm_cuda_i32_f64)."""
+    L = _native.lib()
+    with torch.cuda.device(x.device):
+        rc = L.gl_pbr_spmm_i32_f64(
+            pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
+            meta["codes"].numel(), meta["codes"].data_ptr(), meta["coords"].data_ptr(),
+            meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
+            meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
+            meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), x.shape[1], y.data_ptr(),
+            y.stride(0), 1 if accumulate else 0, _native.stream_handle(x.device))
+    _native.check(rc, "gl_pbr_spmm_i32_f64")
+    return y
+
+
 def pbr_batched_matmul_cuda(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0) -> torch.Tensor:
-    """Y = A @ x with shape [batch, rows, features] (x is [rows, K] or [batch, rows, K])."""
+    """Y = A @ x with shape [batch, rows, features] (x is [rows, K] or [batch, rows, K]).
+    A float64 matrix takes float64 x (the reference's i32_f64 variant); `mode` and `kw`
+    apply to float32."""
     meta = _meta_or_raise(pbr_mat)
-    if meta["data"].dtype != torch.float32:
+    if meta["data"].dtype not in (torch.float32, torch.float64) or x.dtype != meta["data"].dtype:
         raise TypeError(f"Unsupported dtype combination: index={meta['coords'].dtype}, "
-                        f"data={meta['data'].dtype}")
+                        f"data={meta['data'].dtype}, x={x.dtype}")
     x3 = x.unsqueeze(0) if x.dim() == 2 else x
     x3 = x3.contiguous()
     y = torch.empty((x3.shape[0], pbr_mat.rows, x3.shape[2]), dtype=x3.dtype, device=x3.device)

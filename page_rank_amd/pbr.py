@@ -9,7 +9,8 @@ Differences from the reference, all deliberate:
     (``.to()`` to that device returns ``self``), and its host arrays are fetched lazily;
   * ``.to('cuda')`` and ``.to('cuda:0')`` name the same device (the reference compares
     device strings and re-uploads);
-  * the device path is FP32: ``.to()`` of a float64 matrix raises TypeError.
+  * float64 matrices upload as float64 and run the FP64 kernels (SpMM and the reference's
+    PPR loop); the batched seed-set PPR, the warm start and the tuned SpMM plans are FP32.
 """
 from __future__ import annotations
 
@@ -218,9 +219,9 @@ def upload(p: PBRMatrix, device):
     (as __init__.py:43-51), values as stored, plus the stripe index."""
     import torch
     _native.lib()  # fail loudly before any copy if the HIP library is missing
-    if p.scalar_dtype != np.float32:
-        raise TypeError("Only float32 PBR matrices are supported on the GPU path; build the "
-                        "matrix from float32 data")
+    if p.scalar_dtype not in (np.float32, np.float64):
+        raise TypeError(f"Unsupported value dtype {p.scalar_dtype}")
+    vdt = torch.float64 if p.scalar_dtype == np.float64 else torch.float32
     if p.block_rows not in (2, 4, 8) or p.block_rows != p.block_cols:
         raise ValueError("Unsupported block size for PBR SpMM. Only 2x2, 4x4, and 8x8 are "
                          "supported.")
@@ -235,10 +236,10 @@ def upload(p: PBRMatrix, device):
         "codes": torch.as_tensor(np.ascontiguousarray(h["block_codes"]).view(np.int64)).to(device),
         "coords": dev(h["block_coords"], torch.int32),
         "offsets": dev(h["block_offsets"], torch.int32),
-        "data": dev(h["block_data"], torch.float32),
+        "data": dev(h["block_data"], vdt),
         "rem_indptr": dev(h["remainder_indptr"], torch.int32),
         "rem_col_ind": dev(h["remainder_col_ind"], torch.int32),
-        "rem_vals": dev(h["remainder_data"], torch.float32),
+        "rem_vals": dev(h["remainder_data"], vdt),
     }
     meta["stripe_ptr"] = stripe_index(meta["coords"], p.rows, p.block_rows,
                                       meta["codes"].numel(), device)

@@ -79,7 +79,8 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
     if meta["data"].dtype != torch.float32:
-        raise TypeError("run_ppr needs a float32 matrix")
+        raise TypeError("run_ppr computes in float32; a float64 matrix runs through "
+                        "run_personalized_pagerank (the reference loop in FP64) or pbr_matmul")
     if pbr_mat.rows != pbr_mat.cols:
         raise ValueError("PPR needs a square matrix")
     device = meta["codes"].device
@@ -145,6 +146,9 @@ def run_personalized_pagerank(pbr_mat, source_nodes: torch.Tensor, alpha: float 
     meta = get_pbr_gpu_meta(pbr_mat)
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
+    if meta["data"].dtype == torch.float64:
+        return _reference_loop_f64(pbr_mat, meta, source_nodes, alpha, max_iterations,
+                                   tolerance, return_history)
     src = torch.as_tensor(source_nodes).to(torch.int32).cpu().numpy().astype(np.int64)
     res = run_ppr(pbr_mat, [[int(s)] for s in src], alpha=alpha, tol=tolerance,
                   max_iter=max_iterations, dangling_mode="none", retire="together",
@@ -155,3 +159,49 @@ def run_personalized_pagerank(pbr_mat, source_nodes: torch.Tensor, alpha: float 
         history = [float(h) for h in res.history.max(axis=1)]
         return res.X, iterations, converged, history
     return res.X, iterations, converged
+
+
+def _reference_loop_f64(pbr_mat, meta, source_nodes, alpha, max_iterations, tolerance,
+                        return_history):
+    """The reference loop for a float64 matrix (__init__.py:210-271 with its double
+    kernels): one-hot init, then per iteration Y = A X, X = alpha Y + (1 - alpha) e_s with
+    per-column L1 residuals, and a host check of their maximum."""
+    from .ops import spmm_into
+    dev = meta["codes"].device
+    src = torch.as_tensor(source_nodes)
+    if src.dim() != 1 or src.numel() == 0:
+        raise ValueError("source_nodes must be a non-empty 1-D tensor")
+    src_h = src.to(torch.int64).cpu()
+    N, K = pbr_mat.rows, src.numel()
+    if pbr_mat.rows != pbr_mat.cols:
+        raise ValueError("PPR needs a square matrix")
+    if int(src_h.min()) < 0 or int(src_h.max()) >= N:
+        raise ValueError(f"source node out of range [0, {N})")
+    src_d = src_h.to(torch.int32).to(dev)
+    L = _native.lib()
+    X = torch.empty((N, K), dtype=torch.float64, device=dev)
+    Y = torch.empty_like(X)
+    errors = torch.empty(K, dtype=torch.float64, device=dev)
+    wsb = L.gl_ppr_update_f64_workspace_bytes(N, K)
+    ws = torch.empty(max(wsb, 8), dtype=torch.uint8, device=dev)
+    st = _native.stream_handle(dev)
+    with torch.cuda.device(dev):
+        _native.check(L.gl_ppr_init_f64(X.data_ptr(), src_d.data_ptr(), N, K, st),
+                      "gl_ppr_init_f64")
+        history = []
+        converged = False
+        iterations = max_iterations
+        for t in range(max_iterations):
+            spmm_into(pbr_mat, X, Y)
+            _native.check(L.gl_ppr_update_normalized_f64(
+                Y.data_ptr(), X.data_ptr(), src_d.data_ptr(), float(alpha), N, K,
+                errors.data_ptr(), ws.data_ptr(), wsb, st), "gl_ppr_update_normalized_f64")
+            err = float(errors.max().item())
+            history.append(err)
+            if err < tolerance:
+                iterations = t + 1
+                converged = True
+                break
+    if return_history:
+        return X, iterations, converged, history
+    return X, iterations, converged

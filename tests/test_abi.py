@@ -40,7 +40,7 @@ def test_ctypes_table_matches_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 3
+    assert L.gl_abi_version() == 4
     assert L.gl_last_error() is not None
 
 
@@ -68,3 +68,18 @@ def test_gpu_entry_points_fail_loudly_without_library(monkeypatch):
     monkeypatch.setattr(_native, "LIB_PATH", "/nonexistent/libgraphlink_hip.so")
     with pytest.raises(_native.NativeLibraryMissing):
         _native.lib()
+
+
+def test_fp64_argument_validation_without_device():
+    L = _native.lib()
+    rc = L.gl_pbr_spmm_i32_f64(8, 8, 4, 2, 0, None, None, None, None, None, None, None, None,
+                               None, 1, 1, None, 1, 0, None)
+    assert rc == _native.GL_ERR_INVALID and "Unsupported block size" in _native.last_error()
+    rc = L.gl_pbr_spmm_i32_f64(0, 8, 4, 4, 0, None, None, None, None, None, None, None, None,
+                               None, 1, 2, None, 2, 0, None)
+    assert rc == _native.GL_ERR_INVALID and "leading dimension" in _native.last_error()
+    assert L.gl_ppr_init_f64(None, None, 0, 1, None) == _native.GL_ERR_INVALID
+    assert L.gl_ppr_update_f64_workspace_bytes(0, 1) == -1
+    assert L.gl_ppr_update_f64_workspace_bytes(1000, 3) >= 3 * 8
+    rc = L.gl_ppr_update_normalized_f64(1, 1, 1, 0.85, 1000, 3, 1, None, 0, None)
+    assert rc == _native.GL_ERR_INVALID and "workspace too small" in _native.last_error()
