@@ -31,6 +31,19 @@
 #ifndef GL_BIT_PIPE
 #define GL_BIT_PIPE 0
 #endif
+// 8x8 tiles at one RHS column: lane (row r, slot q) walks the 8 bits of row r of tile q of
+// each 8-tile pass (1), instead of one lane per tile bit (0). Measured on MI355X (1M x 1M,
+// 8 tiles per stripe): tiles 76 vs 93 us, the fused SpMM 113.5 vs 128.5 us.
+#ifndef GL_BIT_ROWLANE
+#define GL_BIT_ROWLANE 1
+#endif
+// Bit-lane kernel: load each stripe's first remainder entries a stripe ahead (row spans two
+// ahead), so the stripe's first X-row gathers leave together with its tile loads (1).
+// Measured on MI355X: 7 % slower at 1 column and 12 % at 8 (one wave per SIMD fewer), so it
+// is off.
+#ifndef GL_BIT_REM_AHEAD
+#define GL_BIT_REM_AHEAD 0
+#endif
 // Cache policy of the loads that stream the matrix (each byte read once per pass): 0 =
 // default, 2 = NT (non-temporal). Measured on MI355X, NT was 3-13 % slower (SpMM and the
 // PPR step), so the default policy stays.
@@ -921,6 +934,8 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   constexpr int G = kWave / P;      // tiles per wave
   constexpr int U = UB;
   constexpr int L = kWave / B;      // lanes per matrix row
+  // 8x8 tiles at one column: row lanes (see GL_BIT_ROWLANE)
+  constexpr bool ROWLANE = GL_BIT_ROWLANE && B == 8 && KW == 1;
   __shared__ float red[128];
 
   if (!epi.active()) return;
@@ -931,6 +946,9 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   const int rl = p / B;             // row of this lane inside the stripe
   const int cl = p % B;             // column inside the tile
   const int q = g * B + cl;         // slot among the L lanes of row rl
+  // row-lane tiles: code bits of the rows above row rl (value offset of its first bit)
+  const uint32_t rmask_lo = rl >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rl)) - 1u);
+  const uint32_t rmask_hi = rl >= 4 ? ((1u << (8 * (rl - 4))) - 1u) : 0u;
   const int kwv = (int)min<int64_t>(KW, d.K - d.k0);  // valid columns of this launch
   const bool vec = (d.ldx & 3) == 0 && kwv == KW;  // float4 rows, full column chunk
   epi.lane_init(d.k0 + q, q < kwv);
@@ -980,6 +998,16 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
   if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rl);
+#if GL_BIT_REM_AHEAD
+  RowSpan rs_ahead{0, 0};  // row span of the next stripe
+  if (REM && s + sc.stride < sc.end) rs_ahead = load_row_span(ld, m, (s + sc.stride) * B + rl);
+  int scol_ahead = 0;      // first remainder entry of this lane's slot in the current stripe
+  float sval_ahead = 0.f;
+  if constexpr (REM) {
+    scol_ahead = (int)ld.rcol.u32(rs.j0 + q, q < rs.n);
+    sval_ahead = ld.rval.f32(rs.j0 + q, q < rs.n);
+  }
+#endif
 #if GL_BIT_PIPE
   BitBatch Pb;
   if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
@@ -1000,6 +1028,24 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       tc0 = m.stripe_ptr[snn];
       tc1 = m.stripe_ptr[snn + 1];
     }
+#if GL_BIT_REM_AHEAD
+    rsn = rs_ahead;
+    RowSpan rsnn{0, 0};
+    if (REM && snn < sc.end) rsnn = load_row_span(ld, m, snn * B + rl);
+    int scol = scol_ahead;
+    float sval = sval_ahead;
+    // the next stripe's first entries, and this stripe's first X rows (with the tile loads)
+    int ncol = 0;
+    float nval = 0.f;
+    float xfirst[KW];
+#pragma unroll
+    for (int k = 0; k < KW; ++k) xfirst[k] = 0.f;
+    if constexpr (REM) {
+      ncol = (int)ld.rcol.u32(rsn.j0 + q, q < rsn.n);
+      nval = ld.rval.f32(rsn.j0 + q, q < rsn.n);
+      load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, q < rs.n ? kwv : 0, xfirst);
+    }
+#else
     if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rl);
     // first remainder entry of this lane's row slot
     int scol = 0;
@@ -1009,12 +1055,50 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       scol = (int)ld.rcol.u32(rs.j0 + q, ok);
       sval = ld.rval.f32(rs.j0 + q, ok);
     }
+#endif
 
     float acc[KW];
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = 0.f;
 
-    if constexpr (TILES) {
+    if constexpr (TILES && ROWLANE) {
+      // lane (rl, q): row rl of tile q of each pass of 8 tiles. The row's 8 bits are byte rl
+      // of the code, its values start after the set bits of the rows above it.
+      const bool xwide = BUF && d.ldx == 1;  // X column 0 holds the 8 inputs contiguously
+      for (int tc = ta0; tc < ta1; tc += kWave) {
+        if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
+        const int tce = min(ta1, tc + kWave);
+        for (int tb = tc; tb < tce; tb += B) {
+          const int src = (tb - tc + q) & (kWave - 1);
+          const bool tv = tb + q < tce;
+          const uint32_t clo = __shfl(mt.clo, src, kWave);
+          const uint32_t chi = __shfl(mt.chi, src, kWave);
+          const int c0 = __shfl(mt.c0, src, kWave);
+          const int off = __shfl(mt.off, src, kWave);
+          const uint32_t bits = tv ? ((rl < 4 ? clo >> (8 * rl) : chi >> (8 * (rl - 4))) & 0xFFu)
+                                   : 0u;
+          const int base = off + __popc(clo & rmask_lo) + __popc(chi & rmask_hi);
+          float xv[B];
+          if (xwide && c0 + B <= m.cols) {
+            const float4 a = ld.x.f32x4(c0, tv);
+            const float4 b = ld.x.f32x4(c0 + 4, tv);
+            xv[0] = a.x; xv[1] = a.y; xv[2] = a.z; xv[3] = a.w;
+            xv[4] = b.x; xv[5] = b.y; xv[6] = b.z; xv[7] = b.w;
+          } else {
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+              xv[c] = ld.x.f32(xrow<BUF>((int64_t)c0 + c, d.ldx),
+                               tv && (BUF || (int64_t)c0 + c < m.cols));
+          }
+          float vv[B];
+#pragma unroll
+          for (int c = 0; c < B; ++c)
+            vv[c] = ld.data.f32(base + __popc(bits & ((1u << c) - 1u)), (bits >> c) & 1u);
+#pragma unroll
+          for (int c = 0; c < B; ++c) acc[0] = __builtin_fmaf(vv[c], xv[c], acc[0]);
+        }
+      }
+    } else if constexpr (TILES) {
 #if GL_BIT_PIPE
       // Pb holds the loads of this stripe's first batch, issued during the previous stripe.
       for (int tc = ta0; tc < ta1; tc += kWave) {
@@ -1058,12 +1142,23 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       // the lane's accumulators and are summed with the tile products below
       for (int jb = 0; __any(jb < rs.n); jb += L) {
         const bool ok = jb + q < rs.n;
+        float xv[KW];
+#if GL_BIT_REM_AHEAD
+        if (jb == 0) {
+#pragma unroll
+          for (int k = 0; k < KW; ++k) xv[k] = xfirst[k];
+        } else {
+          scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
+          sval = ld.rval.f32(rs.j0 + jb + q, ok);
+          load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
+        }
+#else
         if (jb > 0) {
           scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
           sval = ld.rval.f32(rs.j0 + jb + q, ok);
         }
-        float xv[KW];
         load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
+#endif
 #pragma unroll
         for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(sval, xv[k], acc[k]);
       }
@@ -1097,6 +1192,11 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     tb1 = tc1;
     mt = mtn;
     rs = rsn;
+#if GL_BIT_REM_AHEAD
+    rs_ahead = rsnn;
+    scol_ahead = ncol;
+    sval_ahead = nval;
+#endif
   }
   epi.finish(q < kwv ? q : -1, red);
 }
