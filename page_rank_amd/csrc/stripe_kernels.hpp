@@ -31,11 +31,18 @@
 #ifndef GL_BIT_PIPE
 #define GL_BIT_PIPE 0
 #endif
-// 8x8 tiles at one RHS column: lane (row r, slot q) walks the 8 bits of row r of tile q of
-// each 8-tile pass (1), instead of one lane per tile bit (0). Measured on MI355X (1M x 1M,
-// 8 tiles per stripe): tiles 76 vs 93 us, the fused SpMM 113.5 vs 128.5 us.
+// Densely tiled 8x8 matrices (the launcher's 8-tile batch) at one RHS column: lane (row r,
+// slot q) walks the 8 bits of row r of tile q of each 8-tile pass (1), instead of one lane
+// per tile bit (0). Measured on MI355X (1M x 1M, 8 tiles per stripe): tiles 76 vs 93 us, the
+// fused SpMM 113.5 vs 128.5 us. On sparse tilings most of the 8 slots would stay empty.
 #ifndef GL_BIT_ROWLANE
 #define GL_BIT_ROWLANE 1
+#endif
+// The same row lanes for 2, 4 and 8 RHS columns (1), or only for one column (0). Measured
+// on MI355X: 16 % slower at 2 columns, equal at 4, 2 % slower at 8 (more registers per lane,
+// one X row per bit instead of per tile), so it is off.
+#ifndef GL_BIT_ROWLANE_WIDE
+#define GL_BIT_ROWLANE_WIDE 0
 #endif
 // Bit-lane kernel: load each stripe's first remainder entries a stripe ahead (row spans two
 // ahead), so the stripe's first X-row gathers leave together with its tile loads (1).
@@ -934,8 +941,9 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   constexpr int G = kWave / P;      // tiles per wave
   constexpr int U = UB;
   constexpr int L = kWave / B;      // lanes per matrix row
-  // 8x8 tiles at one column: row lanes (see GL_BIT_ROWLANE)
-  constexpr bool ROWLANE = GL_BIT_ROWLANE && B == 8 && KW == 1;
+  // densely tiled 8x8 matrices (UB == 8, whole-stripe batches): row lanes (GL_BIT_ROWLANE)
+  constexpr bool ROWLANE =
+      GL_BIT_ROWLANE && B == 8 && UB == 8 && (KW == 1 || GL_BIT_ROWLANE_WIDE);
   __shared__ float red[128];
 
   if (!epi.active()) return;
@@ -1062,8 +1070,9 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     for (int k = 0; k < KW; ++k) acc[k] = 0.f;
 
     if constexpr (TILES && ROWLANE) {
-      // lane (rl, q): row rl of tile q of each pass of 8 tiles. The row's 8 bits are byte rl
-      // of the code, its values start after the set bits of the rows above it.
+      // lane (rl, q): row rl of tile q of each pass of 8 tiles, all KW columns. The row's 8
+      // bits are byte rl of the code, its values start after the set bits of the rows above
+      // it.
       const bool xwide = BUF && d.ldx == 1;  // X column 0 holds the 8 inputs contiguously
       for (int tc = ta0; tc < ta1; tc += kWave) {
         if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
@@ -1078,6 +1087,21 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
           const uint32_t bits = tv ? ((rl < 4 ? clo >> (8 * rl) : chi >> (8 * (rl - 4))) & 0xFFu)
                                    : 0u;
           const int base = off + __popc(clo & rmask_lo) + __popc(chi & rmask_hi);
+          if constexpr (KW > 1) {
+            // X row of each set bit (zeros for the others), times the bit's value
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+              const bool hb = (bits >> c) & 1u;
+              const float v = ld.data.f32(base + __popc(bits & ((1u << c) - 1u)), hb);
+              const int64_t col = (int64_t)c0 + c;
+              float xr[KW];
+              load_row<KW>(ld.x, xrow<BUF>(col, d.ldx), vec,
+                           hb && (BUF || col < m.cols) ? kwv : 0, xr);
+#pragma unroll
+              for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(v, xr[k], acc[k]);
+            }
+            continue;
+          }
           float xv[B];
           if (xwide && c0 + B <= m.cols) {
             const float4 a = ld.x.f32x4(c0, tv);

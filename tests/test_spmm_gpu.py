@@ -180,7 +180,7 @@ def test_block_only_product_named_like_the_reference(device):
 @pytest.mark.parametrize("mode", ["fused", "split"])
 def test_spmm_densely_tiled_8x8(kw, mode, device):
     """8x8 tilings with about ten tiles per stripe (the 1M x 1M workload's shape, which takes
-    the whole-stripe tile batches and, at 32 and 64 columns, the matrix-core product): odd
+    the whole-stripe tile batches, and the row-lane tiles at one column): odd
     stripe count, a partial last stripe, an empty stripe beside a full one, unequal tile
     counts in adjacent stripes, a scattered remainder, strided X and Y, accumulate."""
     n = 8 * 41 + 3
@@ -193,7 +193,7 @@ def test_spmm_densely_tiled_8x8(kw, mode, device):
     pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
     ns = (n + 7) // 8
     assert pbr.accounted_blocks() >= 6 * ns  # the dense-tiling launch
-    for K in (32, 64, 20, 97, 128):
+    for K in (1, 3, 8, 32, 64, 20, 97, 128):
         X = random_x(n, K, seed=K)
         xd = torch.from_numpy(X).to(device)
         Y = graph_link.pbr_matmul(pbr, xd, mode=mode, kw=kw).cpu().numpy()
