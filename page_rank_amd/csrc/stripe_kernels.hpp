@@ -38,19 +38,6 @@
 #ifndef GL_BIT_ROWLANE
 #define GL_BIT_ROWLANE 1
 #endif
-// The same row lanes for 2, 4 and 8 RHS columns (1), or only for one column (0). Measured
-// on MI355X: 16 % slower at 2 columns, equal at 4, 2 % slower at 8 (more registers per lane,
-// one X row per bit instead of per tile), so it is off.
-#ifndef GL_BIT_ROWLANE_WIDE
-#define GL_BIT_ROWLANE_WIDE 0
-#endif
-// Bit-lane kernel: load each stripe's first remainder entries a stripe ahead (row spans two
-// ahead), so the stripe's first X-row gathers leave together with its tile loads (1).
-// Measured on MI355X: 7 % slower at 1 column and 12 % at 8 (one wave per SIMD fewer), so it
-// is off.
-#ifndef GL_BIT_REM_AHEAD
-#define GL_BIT_REM_AHEAD 0
-#endif
 // Cache policy of the loads that stream the matrix (each byte read once per pass): 0 =
 // default, 2 = NT (non-temporal). Measured on MI355X, NT was 3-13 % slower (SpMM and the
 // PPR step), so the default policy stays.
@@ -73,15 +60,6 @@
 #endif
 #ifndef GL_REM_FLAT_EB
 #define GL_REM_FLAT_EB 4
-#endif
-// Feature-lane remainder (row lanes): issue the X-row gathers of the next entry batch, and
-// the column indices of the next block of entries, before the FMAs of the current batch
-// (1), or load and consume one batch at a time (0). Measured on MI355X: 13 % slower for
-// SpMM at 32 columns, 21 % for the 32-column PPR step and 79 % for the 64-column one (the
-// second batch spills under the 128-VGPR cap); with 1- and 2-entry batches, 1-14 % slower.
-// So it is off.
-#ifndef GL_REM_PIPE
-#define GL_REM_PIPE 0
 #endif
 // Remainder entries whose X-row gathers are issued together (feature-lane kernel).
 #ifndef GL_REM_EB
@@ -700,16 +678,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
       sval = ld.rval.f32(rs.j0 + qq, ok);
     }
-#if GL_REM_PIPE
-    // ... and of the second block of L entries
-    int ncol = 0;
-    float nval = 0.f;
-    if constexpr (REM) {
-      const bool ok = L + qq < rs.n;
-      ncol = (int)ld.rcol.u32(rs.j0 + L + qq, ok);
-      nval = ld.rval.f32(rs.j0 + L + qq, ok);
-    }
-#endif
 #endif
 
     float acc[B];
@@ -817,48 +785,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
 #pragma unroll
       for (int i = 0; i < FPL; ++i) racc2[i] = 0.f;
       const int nmax = wave_max(rs.n);
-#if GL_REM_PIPE
-      // Batches of EB entries (EB divides L, so a batch lies inside one block of L entries
-      // whose columns and values are spread over the row's L lanes). Two batch buffers:
-      // the gathers of batch e + EB are in flight while batch e is multiplied.
-      struct RemBatch {
-        float x[EB][FPL];
-        float a[EB];
-      };
-      auto issue = [&](RemBatch& b, int e) {
-        if (e % L == 0 && e > 0) {  // next block: its indices were requested a block ago
-          scol = ncol;
-          sval = nval;
-          const bool ok = e + L + qq < rs.n;
-          ncol = (int)ld.rcol.u32(rs.j0 + e + L + qq, ok);
-          nval = ld.rval.f32(rs.j0 + e + L + qq, ok);
-        }
-#pragma unroll
-        for (int i = 0; i < EB; ++i) {
-          const int c = __shfl(scol, gbase + (e % L) + i, kWave);
-          b.a[i] = __shfl(sval, gbase + (e % L) + i, kWave);
-          const bool ok = e + i < rs.n;
-          load_row<FPL>(ld.x, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, b.x[i]);
-        }
-      };
-      auto consume = [&](const RemBatch& b) {
-#pragma unroll
-        for (int i = 0; i < EB; ++i)
-#pragma unroll
-          for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(b.a[i], b.x[i][u], racc2[u]);
-      };
-      if (nmax > 0) {
-        RemBatch b0, b1;
-        issue(b0, 0);
-        for (int e = 0; e < nmax; e += 2 * EB) {
-          const bool more = e + EB < nmax;
-          if (more) issue(b1, e + EB);
-          consume(b0);
-          if (e + 2 * EB < nmax) issue(b0, e + 2 * EB);
-          if (more) consume(b1);
-        }
-      }
-#else
       for (int jb = 0; jb < nmax; jb += L) {
         if (jb > 0) {
           const bool ok = jb + qq < rs.n;
@@ -881,7 +807,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
             for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(a[i], xv[i][u], racc2[u]);
         }
       }
-#endif
       // transpose through LDS: (row rr, columns kq..) -> (column f, all rows)
       if (kq < KW) {
 #pragma unroll
@@ -942,8 +867,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   constexpr int U = UB;
   constexpr int L = kWave / B;      // lanes per matrix row
   // densely tiled 8x8 matrices (UB == 8, whole-stripe batches): row lanes (GL_BIT_ROWLANE)
-  constexpr bool ROWLANE =
-      GL_BIT_ROWLANE && B == 8 && UB == 8 && (KW == 1 || GL_BIT_ROWLANE_WIDE);
+  constexpr bool ROWLANE = GL_BIT_ROWLANE && B == 8 && UB == 8 && KW == 1;
   __shared__ float red[128];
 
   if (!epi.active()) return;
@@ -1006,16 +930,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
   if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rl);
-#if GL_BIT_REM_AHEAD
-  RowSpan rs_ahead{0, 0};  // row span of the next stripe
-  if (REM && s + sc.stride < sc.end) rs_ahead = load_row_span(ld, m, (s + sc.stride) * B + rl);
-  int scol_ahead = 0;      // first remainder entry of this lane's slot in the current stripe
-  float sval_ahead = 0.f;
-  if constexpr (REM) {
-    scol_ahead = (int)ld.rcol.u32(rs.j0 + q, q < rs.n);
-    sval_ahead = ld.rval.f32(rs.j0 + q, q < rs.n);
-  }
-#endif
 #if GL_BIT_PIPE
   BitBatch Pb;
   if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
@@ -1036,24 +950,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       tc0 = m.stripe_ptr[snn];
       tc1 = m.stripe_ptr[snn + 1];
     }
-#if GL_BIT_REM_AHEAD
-    rsn = rs_ahead;
-    RowSpan rsnn{0, 0};
-    if (REM && snn < sc.end) rsnn = load_row_span(ld, m, snn * B + rl);
-    int scol = scol_ahead;
-    float sval = sval_ahead;
-    // the next stripe's first entries, and this stripe's first X rows (with the tile loads)
-    int ncol = 0;
-    float nval = 0.f;
-    float xfirst[KW];
-#pragma unroll
-    for (int k = 0; k < KW; ++k) xfirst[k] = 0.f;
-    if constexpr (REM) {
-      ncol = (int)ld.rcol.u32(rsn.j0 + q, q < rsn.n);
-      nval = ld.rval.f32(rsn.j0 + q, q < rsn.n);
-      load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, q < rs.n ? kwv : 0, xfirst);
-    }
-#else
     if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rl);
     // first remainder entry of this lane's row slot
     int scol = 0;
@@ -1063,16 +959,14 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       scol = (int)ld.rcol.u32(rs.j0 + q, ok);
       sval = ld.rval.f32(rs.j0 + q, ok);
     }
-#endif
 
     float acc[KW];
 #pragma unroll
     for (int k = 0; k < KW; ++k) acc[k] = 0.f;
 
     if constexpr (TILES && ROWLANE) {
-      // lane (rl, q): row rl of tile q of each pass of 8 tiles, all KW columns. The row's 8
-      // bits are byte rl of the code, its values start after the set bits of the rows above
-      // it.
+      // lane (rl, q): row rl of tile q of each pass of 8 tiles. The row's 8 bits are byte rl
+      // of the code, its values start after the set bits of the rows above it.
       const bool xwide = BUF && d.ldx == 1;  // X column 0 holds the 8 inputs contiguously
       for (int tc = ta0; tc < ta1; tc += kWave) {
         if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
@@ -1087,21 +981,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
           const uint32_t bits = tv ? ((rl < 4 ? clo >> (8 * rl) : chi >> (8 * (rl - 4))) & 0xFFu)
                                    : 0u;
           const int base = off + __popc(clo & rmask_lo) + __popc(chi & rmask_hi);
-          if constexpr (KW > 1) {
-            // X row of each set bit (zeros for the others), times the bit's value
-#pragma unroll
-            for (int c = 0; c < B; ++c) {
-              const bool hb = (bits >> c) & 1u;
-              const float v = ld.data.f32(base + __popc(bits & ((1u << c) - 1u)), hb);
-              const int64_t col = (int64_t)c0 + c;
-              float xr[KW];
-              load_row<KW>(ld.x, xrow<BUF>(col, d.ldx), vec,
-                           hb && (BUF || col < m.cols) ? kwv : 0, xr);
-#pragma unroll
-              for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(v, xr[k], acc[k]);
-            }
-            continue;
-          }
           float xv[B];
           if (xwide && c0 + B <= m.cols) {
             const float4 a = ld.x.f32x4(c0, tv);
@@ -1167,22 +1046,11 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       for (int jb = 0; __any(jb < rs.n); jb += L) {
         const bool ok = jb + q < rs.n;
         float xv[KW];
-#if GL_BIT_REM_AHEAD
-        if (jb == 0) {
-#pragma unroll
-          for (int k = 0; k < KW; ++k) xv[k] = xfirst[k];
-        } else {
-          scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
-          sval = ld.rval.f32(rs.j0 + jb + q, ok);
-          load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
-        }
-#else
         if (jb > 0) {
           scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
           sval = ld.rval.f32(rs.j0 + jb + q, ok);
         }
         load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
-#endif
 #pragma unroll
         for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(sval, xv[k], acc[k]);
       }
@@ -1216,11 +1084,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     tb1 = tc1;
     mt = mtn;
     rs = rsn;
-#if GL_BIT_REM_AHEAD
-    rs_ahead = rsnn;
-    scol_ahead = ncol;
-    sval_ahead = nval;
-#endif
   }
   epi.finish(q < kwv ? q : -1, red);
 }
