@@ -127,9 +127,43 @@ def cpu_baseline(A, K, seconds):
         reps += 1
         t = time.perf_counter() - t0
     gnz = S.nnz * K * reps / t / 1e9
-    return {"value": round(gnz, 4), "unit": "GNZ/s", "cores": 1, "kind": "port",
-            "sample": f"rows 0..{rows} of the matrix ({S.nnz} nnz), K={K}, {reps} products "
-                      f"in {t:.1f} s, oracle/pbr_oracle.cpp spmm (float accumulation)"}
+    out = {"value": round(gnz, 4), "unit": "GNZ/s", "cores": 1, "kind": "port",
+           "sample": f"rows 0..{rows} of the matrix ({S.nnz} nnz), K={K}, {reps} products "
+                     f"in {t:.1f} s, oracle/pbr_oracle.cpp spmm (float accumulation)"}
+    out.update(host_info())
+    out["scipy_csr_f32"] = scipy_spmm(A, K)
+    return out
+
+
+def host_info():
+    """Core count and CPU model of the host the CPU baselines ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"host_cores": os.cpu_count(), "host_cpu": model,
+            "cores_used": f"1 of {os.cpu_count()}"}
+
+
+def scipy_spmm(A, K, reps=3):
+    """scipy's FP32 CSR product A @ X on the whole matrix (single-threaded), median of
+    `reps` after one warm-up: the plain-CSR CPU path beside the PBR one."""
+    import numpy as np
+    X = np.random.default_rng(2).random((A.shape[1], K), dtype=np.float32)
+    A.dot(X)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        A.dot(X)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return {"gnz_s": round(A.nnz * K / t / 1e9, 4), "ms": round(t * 1e3, 1), "K": K,
+            "cores": 1, "sample": f"whole matrix ({A.nnz} nnz), median of {reps}"}
 
 
 def builder_leg(A, T, V, R, dev):
@@ -288,7 +322,8 @@ def ppr_leg(args, dev):
                                "kind": "port",
                                "sample": f"{len(cols)} of the {K} seed sets, FP64 scipy power "
                                          f"iteration to the same tol (oracle/ppr.py power), "
-                                         f"{int(it64.max())} iterations in {t_cpu:.1f} s"}
+                                         f"{int(it64.max())} iterations in {t_cpu:.1f} s",
+                               **host_info()}
         log(f"  FP64 check: L1 {l1}, max abs {mx}; CPU {len(cols)/t_cpu:.3f} PPR/s")
     return out
 
