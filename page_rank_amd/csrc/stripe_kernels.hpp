@@ -583,8 +583,11 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   auto load_batch = [&](FeatBatch& bt, const TileMeta& meta, int tc, int tb, int tce) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int src = (tb - tc + u * G + g) & (kWave - 1);
       const bool tv = tb + u * G + g < tce;
+      // A slot past the chunk's last tile takes that tile's metadata, with its values kept
+      // at 0. Its X loads then need no slot predicate, which the compiler would otherwise
+      // turn into a divergent branch around every load.
+      const int src = min(tb - tc + u * G + g, tce - tc - 1) & (kWave - 1);
       const uint32_t clo = __shfl(meta.clo, src, kWave);
       const uint32_t chi = __shfl(meta.chi, src, kWave);
       const int c0 = __shfl(meta.c0, src, kWave);
@@ -598,7 +601,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       const int64_t xb = xrow<BUF>(c0, d.ldx) + f;
 #pragma unroll
       for (int c = 0; c < B; ++c) {
-        const bool okc = tv && fok && (BUF || (int64_t)c0 + c < m.cols);
+        const bool okc = fok && (BUF || (int64_t)c0 + c < m.cols);
         bt.x[u][c] = ld.x.f32(xb, okc, (uint32_t)c * ldb);
       }
     }
