@@ -5,6 +5,7 @@
 #include "mfma_kernels.hpp"
 #include "stripe_kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 // Matrix-core tile product for 32 / 64 columns of densely tiled 8x8 SpMMs (1 = on).
@@ -25,6 +26,33 @@ int grid_for(int64_t work_items, int wg_per_cu);
 int choose_kw(int64_t K);
 bool valid_kw(int kw);
 
+// One launch of the bit-lane (FEAT false) or feature-lane kernel. The VEC form exists only
+// where a lane's slice of an X row is 4 or more floats: bit-lane kernels of 4 and 8 columns,
+// feature-lane remainders with KW / (64 / B) >= 4.
+template <int B, int KW, bool FEAT, bool TILES, bool REM, bool BUF, class Epi, int U>
+void launch_vec(const dim3& grid, hipStream_t st, const PbrView& m, const RhsView& d,
+                const Epi& e, bool vec) {
+  constexpr bool useful = FEAT ? KW / (kWave / B) >= 4 : KW >= 4;
+  const dim3 block(256);
+  if constexpr (useful) {
+    if (vec) {
+      if constexpr (FEAT)
+        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, true>), grid, block, 0,
+                           st, m, d, e);
+      else
+        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, true>), grid, block, 0,
+                           st, m, d, e);
+      return;
+    }
+  }
+  if constexpr (FEAT)
+    hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, false>), grid, block, 0, st,
+                       m, d, e);
+  else
+    hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, false>), grid, block, 0, st,
+                       m, d, e);
+}
+
 template <int B, class Epi, bool TILES, bool REM, bool BUF>
 int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
                      hipStream_t st) {
@@ -42,24 +70,22 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
       return GL_OK;
     }
   }
+  // float4 X rows and a full column chunk: the VEC kernels (no runtime branch per X load)
+  const bool vec = (d.ldx & 3) == 0 && std::min<int64_t>(kw, d.K - d.k0) == kw;
   switch (kw) {
 #define GL_KB(KW)                                                                             \
     case KW:                                                                                  \
       if (wide)                                                                               \
-        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, (B == 8 ? 8 : 2)>),     \
-                           grid, block, 0, st, m, d, e);                                      \
+        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 8 : 2)>(grid, st, m, d, e, vec); \
       else                                                                                    \
-        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, (B == 8 ? 4 : 2)>),     \
-                           grid, block, 0, st, m, d, e);                                      \
+        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 4 : 2)>(grid, st, m, d, e, vec); \
       break;
 #define GL_KF(KW)                                                                             \
     case KW:                                                                                  \
       if (wide)                                                                               \
-        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, 4>), grid, block, 0, st, \
-                           m, d, e);                                                          \
+        launch_vec<B, KW, true, TILES, REM, BUF, Epi, 4>(grid, st, m, d, e, vec);             \
       else                                                                                    \
-        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, 1>), grid, block, 0, st, \
-                           m, d, e);                                                          \
+        launch_vec<B, KW, true, TILES, REM, BUF, Epi, 1>(grid, st, m, d, e, vec);             \
       break;
     GL_KB(1) GL_KB(2) GL_KB(4) GL_KB(8)
     GL_KF(16) GL_KF(32) GL_KF(64)

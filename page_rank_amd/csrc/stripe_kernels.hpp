@@ -512,8 +512,9 @@ __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const Pbr
 // --------------------------------------------------------------------------------------
 
 // Feature-lane kernel, KW in {16, 32, 64}. UF: tile groups whose loads are issued together
-// (the launcher picks 4 for 8x8 matrices with about 8 tiles per stripe, else 1).
-template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UF>
+// (the launcher picks 4 for 8x8 matrices with about 8 tiles per stripe, else 1). VEC: X rows
+// are float4-aligned and the launch's column chunk is full.
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UF, bool VEC>
 __global__ void __launch_bounds__(256)
 #if GL_FEAT_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(GL_FEAT_WAVES)))
@@ -552,7 +553,9 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const bool fok = colk < d.K;
   epi.lane_init(colk, fok);
   const int kvalid = (int)min<int64_t>(KW, d.K - d.k0);
-  const bool vec = (d.ldx & 3) == 0 && kvalid == KW;  // float4 rows, full column chunk
+  // float4 rows and a full column chunk: the launcher's VEC, a compile-time constant, so the
+  // X-row loads carry no runtime branch between the float4 and the per-element form
+  const bool vec = VEC;
   const uint32_t ldb = (uint32_t)d.ldx * 4u;  // bytes per X row
   const int rr = lane / L;                // remainder: row of this lane
   const int qq = lane % L;                // remainder: slot in the row
@@ -862,8 +865,8 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
 // --------------------------------------------------------------------------------------
 // UB: tile groups whose loads are issued together (the launcher picks 8 for 8x8 matrices
 // with about 8 tiles per stripe, where one batch then covers a stripe, else 4; 2 for
-// smaller blocks).
-template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UB>
+// smaller blocks). VEC as for the feature-lane kernel.
+template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UB, bool VEC>
 __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi epi) {
   constexpr int P = B * B;
   constexpr int G = kWave / P;      // tiles per wave
@@ -885,7 +888,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   const uint32_t rmask_lo = rl >= 4 ? 0xFFFFFFFFu : ((1u << (8 * rl)) - 1u);
   const uint32_t rmask_hi = rl >= 4 ? ((1u << (8 * (rl - 4))) - 1u) : 0u;
   const int kwv = (int)min<int64_t>(KW, d.K - d.k0);  // valid columns of this launch
-  const bool vec = (d.ldx & 3) == 0 && kwv == KW;  // float4 rows, full column chunk
+  const bool vec = VEC;  // float4 rows, full column chunk (compile-time, as above)
   epi.lane_init(d.k0 + q, q < kwv);
 
   // Value and X row of this lane's bit in each of U tiles.
