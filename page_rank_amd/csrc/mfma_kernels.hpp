@@ -122,13 +122,13 @@ __global__ void __launch_bounds__(256) k_stripe_mfma(PbrView m, RhsView d, EpiSt
       float racc2[FPL];
 #pragma unroll
       for (int i = 0; i < FPL; ++i) racc2[i] = 0.f;
-      const int nmax = wave_max(rs.n);
+      const int nmax = wave_max(rs.n());
       for (int jb = 0; jb < nmax; jb += EB) {
         float xv[EB][FPL];
         float a[EB];
 #pragma unroll
         for (int i = 0; i < EB; ++i) {
-          const bool ok = jb + i < rs.n;
+          const bool ok = jb + i < rs.n();
           const int cidx = (int)ld.rcol.u32(rs.j0 + jb + i, ok);
           a[i] = ld.rval.f32(rs.j0 + jb + i, ok);
           load_row<FPL>(ld.x, xrow<BUF>(cidx, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);

@@ -490,17 +490,22 @@ __device__ __forceinline__ TileMeta load_tile_meta(const PbrLd<BUF>& ld, int t, 
   return r;
 }
 
-// Remainder entries of one matrix row: [j0, j0 + n).
+// Remainder entries of one matrix row: [j0, j1). The length is taken where it is used: a
+// subtraction right after the two loads makes the wave wait for every load in flight
+// (measured: 2-4.5 % slower SpMM at 2-64 columns and PPR steps). EAGER keeps that early
+// wait, which the one-column bit-lane kernel measured 6 % faster with.
 struct RowSpan {
-  int j0, n;
+  int j0, j1;
+  __device__ __forceinline__ int n() const { return j1 - j0; }
 };
-template <bool BUF>
+template <bool EAGER = false, bool BUF>
 __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const PbrView& m,
-                                                 int64_t row) {
-  const bool ok = row < m.rows;
+                                                 int64_t row, bool pred = true) {
+  const bool ok = pred && row < m.rows;
   RowSpan r;
   r.j0 = (int)ld.rptr.u32(row, ok);
-  r.n = (int)ld.rptr.u32(row + 1, ok) - r.j0;
+  r.j1 = (int)ld.rptr.u32(row + 1, ok);
+  if constexpr (EAGER) asm volatile("" : "+v"(r.j0), "+v"(r.j1));
   return r;
 }
 
@@ -629,7 +634,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   int rsv = 0;
   if (REM && s < sc.end) rsv = (int)ld.rptr.u32(min<int64_t>(s * B + lane, m.rows), lane <= B);
 #else
-  if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rr);
+  if constexpr (REM) rs = load_row_span(ld, m, s * B + rr, s < sc.end);
 #endif
 #if GL_FEAT_PIPE
   FeatBatch Pb;
@@ -675,12 +680,12 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       }
     }
 #else
-    if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rr);
+    if constexpr (REM) rsn = load_row_span(ld, m, sn * B + rr, sn < sc.end);
     // ---- first remainder entries of this stripe
     int scol = 0;
     float sval = 0.f;
     if constexpr (REM) {
-      const bool ok = qq < rs.n;
+      const bool ok = qq < rs.n();
       scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
       sval = ld.rval.f32(rs.j0 + qq, ok);
     }
@@ -790,10 +795,10 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       float racc2[FPL];
 #pragma unroll
       for (int i = 0; i < FPL; ++i) racc2[i] = 0.f;
-      const int nmax = wave_max(rs.n);
+      const int nmax = wave_max(rs.n());
       for (int jb = 0; jb < nmax; jb += L) {
         if (jb > 0) {
-          const bool ok = jb + qq < rs.n;
+          const bool ok = jb + qq < rs.n();
           scol = (int)ld.rcol.u32(rs.j0 + jb + qq, ok);
           sval = ld.rval.f32(rs.j0 + jb + qq, ok);
         }
@@ -804,7 +809,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
           for (int i = 0; i < EB; ++i) {
             const int c = __shfl(scol, gbase + ib + i, kWave);
             a[i] = __shfl(sval, gbase + ib + i, kWave);
-            const bool ok = jb + ib + i < rs.n;
+            const bool ok = jb + ib + i < rs.n();
             load_row<FPL>(ld.x, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
           }
 #pragma unroll
@@ -935,7 +940,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   TileMeta mt{0u, 0u, 0, 0};
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
-  if (REM && s < sc.end) rs = load_row_span(ld, m, s * B + rl);
+  if constexpr (REM) rs = load_row_span<KW == 1>(ld, m, s * B + rl, s < sc.end);
 #if GL_BIT_PIPE
   BitBatch Pb;
   if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
@@ -956,12 +961,12 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       tc0 = m.stripe_ptr[snn];
       tc1 = m.stripe_ptr[snn + 1];
     }
-    if (REM && sn < sc.end) rsn = load_row_span(ld, m, sn * B + rl);
+    if constexpr (REM) rsn = load_row_span<KW == 1>(ld, m, sn * B + rl, sn < sc.end);
     // first remainder entry of this lane's row slot
     int scol = 0;
     float sval = 0.f;
     if constexpr (REM) {
-      const bool ok = q < rs.n;
+      const bool ok = q < rs.n();
       scol = (int)ld.rcol.u32(rs.j0 + q, ok);
       sval = ld.rval.f32(rs.j0 + q, ok);
     }
@@ -1049,8 +1054,8 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     if constexpr (REM) {
       // lane (rl, q) takes entries q, q+L, ... of its own row: products go straight into
       // the lane's accumulators and are summed with the tile products below
-      for (int jb = 0; __any(jb < rs.n); jb += L) {
-        const bool ok = jb + q < rs.n;
+      for (int jb = 0; __any(jb < rs.n()); jb += L) {
+        const bool ok = jb + q < rs.n();
         float xv[KW];
         if (jb > 0) {
           scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
