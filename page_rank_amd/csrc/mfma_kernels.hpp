@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) k_stripe_mfma(PbrView m, RhsView d, EpiSt
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int fe = f * 16 + 4 * kk + r;
-            if (fe < kvalid) epi.put(0, row, d.k0 + fe, c[f][r]);
+            if (fe < kvalid) epi.template put<BUF>(0, row, d.k0 + fe, c[f][r]);
           }
       }
     }

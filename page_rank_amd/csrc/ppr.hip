@@ -363,7 +363,7 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
       EpiPpr e{xc, xn, K, opts->alpha, w.cu, w.tele, w.frozen,
                opts->dangling_mode != GL_DANGLING_NONE ? dangling_mask : nullptr,
                w.sptr, w.srow, w.scol, w.sw, w.ctl, w.part_res, w.part_dang, K, k0,
-               fits_u24(rows, K), 0.f, 0.f, 0, 0};
+               fits_u24(rows, K), rows * K, 0.f, 0.f, 0, 0};
       int rc = launch_stripe<EpiPpr, true, true>(m, d, block_rows, kw, e, nwg, st);
       if (rc) return rc;
     }

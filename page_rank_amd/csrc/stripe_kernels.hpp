@@ -289,7 +289,9 @@ struct EpiStore {
   __device__ __forceinline__ bool active() const { return true; }
   __device__ __forceinline__ void lane_init(int64_t, bool) {}
   __device__ __forceinline__ void begin_stripe(int64_t) {}
+  template <bool BUF>
   __device__ __forceinline__ void prefetch(int, int64_t, int64_t, bool) {}
+  template <bool BUF>
   __device__ __forceinline__ void put(int, int64_t row, int64_t col, float acc) {
     float* p = y + elem(row, ldy, col, u24);
     if (accumulate) *p += acc;
@@ -333,6 +335,7 @@ struct EpiPpr {
   int64_t K;
   int64_t k0;
   bool u24;
+  int64_t xlen;    // elements of xcur (and of xnext)
   // per-lane state
   float res;
   float dang;
@@ -352,12 +355,24 @@ struct EpiPpr {
     e0 = sptr[s];
     e1 = sptr[s + 1];
   }
-  // Issued at the start of a stripe, so X_t arrives while the stripe is multiplied.
-  __device__ __forceinline__ void prefetch(int i, int64_t row, int64_t col, bool ok) {
-    xc[i] = ok ? xcur[elem(row, ld, col, u24)] : 0.f;
+  // Element offset of (row, col) in X_t / X_{t+1}. With BUF (the buffer-descriptor
+  // kernels, whose launch guarantees arrays below 2 GB and rows, ld below 2^24) it is a
+  // 24-bit multiply with no runtime test.
+  template <bool BUF>
+  __device__ __forceinline__ int64_t at(int64_t row, int64_t col) const {
+    if constexpr (BUF) return (int64_t)__umul24((uint32_t)row, (uint32_t)ld) + col;
+    else return elem(row, ld, col, u24);
   }
+  // Issued at the start of a stripe, so X_t arrives while the stripe is multiplied. With BUF
+  // a predicated buffer load (no branch around it).
+  template <bool BUF>
+  __device__ __forceinline__ void prefetch(int i, int64_t row, int64_t col, bool ok) {
+    if constexpr (BUF) xc[i] = Ld<true>(xcur, xlen * 4).f32(at<true>(row, col), ok);
+    else xc[i] = ok ? xcur[at<false>(row, col)] : 0.f;
+  }
+  template <bool BUF>
   __device__ __forceinline__ void put(int i, int64_t row, int64_t col, float acc) {
-    const int64_t ix = elem(row, ld, col, u24);
+    const int64_t ix = at<BUF>(row, col);
     const float x0 = xc[i];
     float xn = x0;
     if (!frz) {
@@ -649,7 +664,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     epi.begin_stripe(s);
 #pragma unroll
     for (int i = 0; i < B / G; ++i)
-      epi.prefetch(i, r0 + i * G + g, colk, fok && r0 + i * G + g < m.rows);
+      epi.template prefetch<BUF>(i, r0 + i * G + g, colk, fok && r0 + i * G + g < m.rows);
     // ---- prefetch for the next stripe
     TileMeta mtn{0u, 0u, 0, 0};
     int tc0 = 0, tc1 = 0;
@@ -847,7 +862,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
 #pragma unroll
       for (int r = 0; r < B; ++r) {
         const int64_t row = r0 + r;
-        if ((r % G) == g && row < m.rows) epi.put(r / G, row, colk, acc[r]);
+        if ((r % G) == g && row < m.rows) epi.template put<BUF>(r / G, row, colk, acc[r]);
       }
     }
     // ---- rotate the pipeline
@@ -952,7 +967,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     const int64_t sn = s + sc.stride;
     const int64_t snn = sn + sc.stride;
     epi.begin_stripe(s);
-    epi.prefetch(0, r0 + rl, d.k0 + q, q < kwv && r0 + rl < m.rows);
+    epi.template prefetch<BUF>(0, r0 + rl, d.k0 + q, q < kwv && r0 + rl < m.rows);
     TileMeta mtn{0u, 0u, 0, 0};
     int tc0 = 0, tc1 = 0;
     RowSpan rsn{0, 0};
@@ -1087,7 +1102,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
 #pragma unroll
       for (int k = 1; k < KW; ++k)
         if (q == k) mine = acc[k];
-      epi.put(0, row, d.k0 + q, mine);
+      epi.template put<BUF>(0, row, d.k0 + q, mine);
     }
     ta0 = tb0;
     ta1 = tb1;
