@@ -99,6 +99,7 @@ __global__ void k_ctl_init(PprCtl* ctl, ColState c, int64_t K, float alpha, int6
 // consecutive threads take consecutive columns (coalesced rows of the partial arrays) and
 // kRetireThreads / KC slices of threads split the workgroup rows; an LDS tree in fixed order
 // combines the slices, so the sums are deterministic.
+constexpr int kUnroll = 8;
 __global__ void __launch_bounds__(kRetireThreads)
 k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ part_res,
          const float* __restrict__ part_dang, float* __restrict__ hist, float tol, float alpha,
@@ -124,9 +125,27 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
     const int64_t k = k0 + kk;
     double r = 0.0, dg = 0.0;
     if (k < K && !c.frozen[k]) {
-      for (int w = sl; w < nwg; w += S) {
-        r += part_res[(int64_t)w * K + k];
-        dg += part_dang[(int64_t)w * K + k];
+      // kUnroll independent accumulators, so that many partial loads are in flight (each
+      // comes from another XCD's writes), combined in a fixed order
+      double ra[kUnroll], da[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) ra[u] = da[u] = 0.0;
+      int w = sl;
+      for (; w + (kUnroll - 1) * S < nwg; w += kUnroll * S) {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          ra[u] += part_res[(int64_t)(w + u * S) * K + k];
+          da[u] += part_dang[(int64_t)(w + u * S) * K + k];
+        }
+      }
+      for (; w < nwg; w += S) {
+        ra[0] += part_res[(int64_t)w * K + k];
+        da[0] += part_dang[(int64_t)w * K + k];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        r += ra[u];
+        dg += da[u];
       }
     }
     s_r[threadIdx.x] = r;
