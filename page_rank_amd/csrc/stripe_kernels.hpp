@@ -732,16 +732,21 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       }
       if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
 #else
-      for (int tc = ta0; tc < ta1; tc += kWave) {
-        if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
+      // Chunks of 64 tiles. The first uses the metadata prefetched a stripe ago; later ones
+      // (stripes of more than 64 tiles) load their own, in a loop of their own, so the
+      // first chunk never waits on a metadata load of this stripe.
+      auto chunk = [&](const TileMeta& meta, int tc) {
         const int tce = min(ta1, tc + kWave);
         for (int tb = tc; tb < tce; tb += G * U) {
           FeatBatch P1;
-          load_batch(P1, mt, tc, tb, tce);
+          load_batch(P1, meta, tc, tb, tce);
 #pragma unroll
           for (int u = 0; u < U; ++u) TileFma<B>::run(acc, P1.v[u], P1.x[u]);
         }
-      }
+      };
+      if (ta0 < ta1) chunk(mt, ta0);
+      for (int tc = ta0 + kWave; tc < ta1; tc += kWave)
+        chunk(load_tile_meta(ld, tc, ta1, lane), tc);
 #endif
     }
 
@@ -994,6 +999,9 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       // lane (rl, q): row rl of tile q of each pass of 8 tiles. The row's 8 bits are byte rl
       // of the code, its values start after the set bits of the rows above it.
       const bool xwide = BUF && d.ldx == 1;  // X column 0 holds the 8 inputs contiguously
+      // The chunk loop keeps its in-loop metadata reload and the per-lane float4 test: for
+      // this kernel, peeling the first chunk and hoisting the test (as the other paths do)
+      // measured 7 % slower.
       for (int tc = ta0; tc < ta1; tc += kWave) {
         if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
         const int tce = min(ta1, tc + kWave);
@@ -1051,18 +1059,21 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       }
       if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
 #else
-      for (int tc = ta0; tc < ta1; tc += kWave) {
-        if (tc != ta0) mt = load_tile_meta(ld, tc, ta1, lane);
+      // chunks of 64 tiles, the first with the prefetched metadata (as in the feature kernel)
+      auto chunk = [&](const TileMeta& meta, int tc) {
         const int tce = min(ta1, tc + kWave);
         for (int tb = tc; tb < tce; tb += G * U) {
           BitBatch P1;
-          load_batch(P1, mt, tc, tb, tce);
+          load_batch(P1, meta, tc, tb, tce);
 #pragma unroll
           for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(P1.v[u], P1.x[u][k], acc[k]);
         }
-      }
+      };
+      if (ta0 < ta1) chunk(mt, ta0);
+      for (int tc = ta0 + kWave; tc < ta1; tc += kWave)
+        chunk(load_tile_meta(ld, tc, ta1, lane), tc);
 #endif
     }
 
