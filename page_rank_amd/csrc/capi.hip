@@ -66,6 +66,12 @@ hipStream_t side_stream() { return per_device().side; }
 hipStream_t capture_stream() { return per_device().cap; }
 hipEvent_t lib_event(int i) { return per_device().ev[i & 3]; }
 
+// SpMM grids: up to 32 workgroups per CU, more than the device holds at once, so the
+// hardware hands out the later ones as earlier ones finish and every CU stays busy to the
+// end. Measured on MI355X (1M x 1M, 8x8) against 8 per CU: 1 column 114 -> 106 us,
+// 2 columns 161 -> 155 us, 64 columns 479 -> 467 us, 8 and 32 columns unchanged.
+constexpr int kSpmmWgPerCu = 32;
+
 int grid_for(int64_t work_items, int wg_per_cu) {
   const int cus = device_info().num_cus;
   int64_t want = (work_items + 3) / 4;  // 4 waves per workgroup, one item per wave
@@ -178,7 +184,7 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
   if (rows == 0) return GL_OK;
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, 0,
             codes, coords, offsets, data, stripe_ptr, nullptr, nullptr, nullptr};
-  const int nwg = grid_for(m.num_stripes, 0);
+  const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
@@ -203,7 +209,7 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* 
   // Row groups of 8 rows; one wave owns a group (the kernel's "stripe").
   PbrView m{rows, cols, (rows + 7) / 8, 0, 0, nnz, nullptr, nullptr, nullptr, nullptr, nullptr,
             indptr, col_ind, vals};
-  const int nwg = grid_for(m.num_stripes, 0);
+  const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
@@ -250,7 +256,7 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
   }
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, num_rem,
             codes, coords, offsets, data, stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
-  const int nwg = grid_for(m.num_stripes, 0);
+  const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
   for (int64_t k0 = 0; k0 < K; k0 += kw) {
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};

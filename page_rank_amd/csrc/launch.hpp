@@ -29,63 +29,80 @@ bool valid_kw(int kw);
 // One launch of the bit-lane (FEAT false) or feature-lane kernel. The VEC form exists only
 // where a lane's slice of an X row is 4 or more floats: bit-lane kernels of 4 and 8 columns,
 // feature-lane remainders with KW / (64 / B) >= 4.
+// Workgroups of `kernel` (256 threads) that one CU holds at once.
+template <class F>
+int resident_per_cu(F kernel) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess) n = 0;
+  return n;
+}
+
+// One launch of the bit-lane (FEAT false) or feature-lane kernel, or, with `occ` set, no
+// launch: *occ = workgroups per CU of the kernel that would run. The VEC form exists only
+// where a lane's slice of an X row is 4 or more floats: bit-lane kernels of 4 and 8 columns,
+// feature-lane remainders with KW / (64 / B) >= 4.
+#define GL_GO(KERN)                                                        \
+  do {                                                                     \
+    static int occ_cached = -1;                                            \
+    if (occ) {                                                             \
+      if (occ_cached < 0) occ_cached = resident_per_cu(KERN);              \
+      *occ = occ_cached;                                                   \
+    } else {                                                               \
+      hipLaunchKernelGGL(KERN, grid, block, 0, st, m, d, e);               \
+    }                                                                      \
+  } while (0)
 template <int B, int KW, bool FEAT, bool TILES, bool REM, bool BUF, class Epi, int U>
 void launch_vec(const dim3& grid, hipStream_t st, const PbrView& m, const RhsView& d,
-                const Epi& e, bool vec) {
+                const Epi& e, bool vec, int* occ) {
   constexpr bool useful = FEAT ? KW / (kWave / B) >= 4 : KW >= 4;
   const dim3 block(256);
   if constexpr (useful) {
     if (vec) {
-      if constexpr (FEAT)
-        hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, true>), grid, block, 0,
-                           st, m, d, e);
-      else
-        hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, true>), grid, block, 0,
-                           st, m, d, e);
+      if constexpr (FEAT) GL_GO((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, true>));
+      else GL_GO((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, true>));
       return;
     }
   }
-  if constexpr (FEAT)
-    hipLaunchKernelGGL((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, false>), grid, block, 0, st,
-                       m, d, e);
-  else
-    hipLaunchKernelGGL((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, false>), grid, block, 0, st,
-                       m, d, e);
+  if constexpr (FEAT) GL_GO((k_stripe_feat<B, KW, TILES, REM, BUF, Epi, U, false>));
+  else GL_GO((k_stripe_bit<B, KW, TILES, REM, BUF, Epi, U, false>));
 }
 
 template <int B, class Epi, bool TILES, bool REM, bool BUF>
 int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
-                     hipStream_t st) {
+                     hipStream_t st, int* occ) {
   const dim3 grid(nwg), block(256);
   // tile batch: a whole stripe at once when 8x8 stripes hold about 8 tiles (measured on
-  // MI355X: -16 % at 1 column, -3 % at 32; narrower batches win on sparser tilings)
+  // MI355X: -16 % at 1 column, -3 % at 32; narrower batches win on sparser tilings). For the
+  // feature-lane kernels that is 8 / (64 / KW) groups, 2 at 16 columns (8 columns computed
+  // with kw 16: 234 -> 196 us against 4 groups) and 4 at 32; 64 columns stay at 4 (registers).
   const bool wide = B == 8 && m.num_tiles >= 6 * m.num_stripes;
   if constexpr (GL_MFMA && B == 8 && TILES && std::is_same<Epi, EpiStore>::value) {
     if (wide && (kw == 32 || kw == 64)) {
-      if (kw == 32)
-        hipLaunchKernelGGL((k_stripe_mfma<32, REM, BUF>), grid, block, 0, st, m, d, e);
-      else
-        hipLaunchKernelGGL((k_stripe_mfma<64, REM, BUF>), grid, block, 0, st, m, d, e);
-      GL_LAUNCH_CHECK("k_stripe_mfma");
+      if (kw == 32) GL_GO((k_stripe_mfma<32, REM, BUF>));
+      else GL_GO((k_stripe_mfma<64, REM, BUF>));
+      if (!occ) GL_LAUNCH_CHECK("k_stripe_mfma");
       return GL_OK;
     }
   }
   // float4 X rows and a full column chunk: the VEC kernels (no runtime branch per X load)
   const bool vec = (d.ldx & 3) == 0 && std::min<int64_t>(kw, d.K - d.k0) == kw;
   switch (kw) {
-#define GL_KB(KW)                                                                             \
-    case KW:                                                                                  \
-      if (wide)                                                                               \
-        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 8 : 2)>(grid, st, m, d, e, vec); \
-      else                                                                                    \
-        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 4 : 2)>(grid, st, m, d, e, vec); \
+#define GL_KB(KW)                                                                        \
+    case KW:                                                                             \
+      if (wide)                                                                          \
+        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 8 : 2)>(grid, st, m, d, e, \
+                                                                         vec, occ);      \
+      else                                                                               \
+        launch_vec<B, KW, false, TILES, REM, BUF, Epi, (B == 8 ? 4 : 2)>(grid, st, m, d, e, \
+                                                                         vec, occ);      \
       break;
-#define GL_KF(KW)                                                                             \
-    case KW:                                                                                  \
-      if (wide)                                                                               \
-        launch_vec<B, KW, true, TILES, REM, BUF, Epi, 4>(grid, st, m, d, e, vec);             \
-      else                                                                                    \
-        launch_vec<B, KW, true, TILES, REM, BUF, Epi, 1>(grid, st, m, d, e, vec);             \
+#define GL_KF(KW)                                                                        \
+    case KW:                                                                             \
+      if (wide)                                                                          \
+        launch_vec<B, KW, true, TILES, REM, BUF, Epi, (KW == 16 ? 2 : 4)>(grid, st, m, d, e, \
+                                                                          vec, occ);     \
+      else                                                                               \
+        launch_vec<B, KW, true, TILES, REM, BUF, Epi, 1>(grid, st, m, d, e, vec, occ);   \
       break;
     GL_KB(1) GL_KB(2) GL_KB(4) GL_KB(8)
     GL_KF(16) GL_KF(32) GL_KF(64)
@@ -95,7 +112,7 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
       set_error("unsupported kw %d", kw);
       return GL_ERR_INVALID;
   }
-  GL_LAUNCH_CHECK("k_stripe");
+  if (!occ) GL_LAUNCH_CHECK("k_stripe");
   return GL_OK;
 }
 
@@ -109,22 +126,37 @@ inline bool buffer_ok(const PbrView& m, const RhsView& d) {
 
 template <int B, class Epi, bool TILES, bool REM>
 int launch_stripe_b(const PbrView& m, const RhsView& d, int kw, const Epi& e, int nwg,
-                    hipStream_t st) {
-  if (buffer_ok(m, d)) return launch_stripe_bb<B, Epi, TILES, REM, true>(m, d, kw, e, nwg, st);
-  return launch_stripe_bb<B, Epi, TILES, REM, false>(m, d, kw, e, nwg, st);
+                    hipStream_t st, int* occ) {
+  if (buffer_ok(m, d))
+    return launch_stripe_bb<B, Epi, TILES, REM, true>(m, d, kw, e, nwg, st, occ);
+  return launch_stripe_bb<B, Epi, TILES, REM, false>(m, d, kw, e, nwg, st, occ);
 }
 
 template <class Epi, bool TILES, bool REM>
 int launch_stripe(const PbrView& m, const RhsView& d, int B, int kw, const Epi& e, int nwg,
-                  hipStream_t st) {
+                  hipStream_t st, int* occ = nullptr) {
   switch (B) {
-    case 2: return launch_stripe_b<2, Epi, TILES, REM>(m, d, kw, e, nwg, st);
-    case 4: return launch_stripe_b<4, Epi, TILES, REM>(m, d, kw, e, nwg, st);
-    case 8: return launch_stripe_b<8, Epi, TILES, REM>(m, d, kw, e, nwg, st);
+    case 2: return launch_stripe_b<2, Epi, TILES, REM>(m, d, kw, e, nwg, st, occ);
+    case 4: return launch_stripe_b<4, Epi, TILES, REM>(m, d, kw, e, nwg, st, occ);
+    case 8: return launch_stripe_b<8, Epi, TILES, REM>(m, d, kw, e, nwg, st, occ);
     default:
       set_error("Unsupported block size %d: only 2x2, 4x4 and 8x8 are supported", B);
       return GL_ERR_INVALID;
   }
+}
+
+#undef GL_GO
+
+// Persistent grid for the kernel that launch_stripe would run: `nwg` workgroups, cut to
+// what the device holds at once, so that no workgroup waits for another to end before it
+// starts its stripes.
+template <class Epi, bool TILES, bool REM>
+int stripe_grid(const PbrView& m, const RhsView& d, int B, int kw, const Epi& e, int nwg) {
+  int occ = 0;
+  if (launch_stripe<Epi, TILES, REM>(m, d, B, kw, e, nwg, nullptr, &occ) || occ < 1) return nwg;
+  int64_t cap = (int64_t)occ * device_info().num_cus;
+  cap -= cap % kNumXcd;
+  return cap >= kNumXcd && cap < nwg ? (int)cap : nwg;
 }
 
 }  // namespace gl

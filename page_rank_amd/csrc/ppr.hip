@@ -370,6 +370,15 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   PbrView m{rows, cols, ns, num_tiles, num_vals, num_rem, codes, coords, offsets, data,
             stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
   int nwg = grid_for(ns, opts->grid_mult);
+  if (opts->grid_mult == 0) {
+    // default grid: what the device holds of the kernel at once
+    const EpiPpr e0{X_out, w.xb, K, opts->alpha, w.cu, w.tele, w.frozen,
+                    opts->dangling_mode != GL_DANGLING_NONE ? dangling_mask : nullptr,
+                    w.sptr, w.srow, w.scol, w.sw, w.ctl, w.part_res, w.part_dang, K, 0,
+                    fits_u24(rows, K), rows * K, 0.f, 0.f, 0, 0};
+    nwg = stripe_grid<EpiPpr, true, true>(m, RhsView{X_out, K, K, 0, rows * K}, block_rows, kw,
+                                          e0, nwg);
+  }
   if (nwg > kMaxWg) nwg = kMaxWg - (kMaxWg % kNumXcd);
   const int together = opts->retire_mode == GL_RETIRE_TOGETHER ? 1 : 0;
 

@@ -78,6 +78,14 @@
 #ifndef GL_FEAT64_WAVES
 #define GL_FEAT64_WAVES 4
 #endif
+// Cache policy of the remainder's X-row gathers (0 = default, 2 = non-temporal, 16 / 17 =
+// device / system scope). Measured on MI355X: non-temporal 13 % slower at 32 columns, 2x for
+// the one-column remainder and 59 % for the 64-column PPR step (X is served from the
+// Infinity Cache, which these loads then skip); 16 and 17 11 % slower at 2 and 8 columns and
+// 32 % for the 64-column PPR step. Default.
+#ifndef GL_REM_X_CP
+#define GL_REM_X_CP 0
+#endif
 // Non-temporal stores of finished output rows (1 = on; measured slower, off by default).
 #ifndef GL_NT_STORE
 #define GL_NT_STORE 0
@@ -180,6 +188,7 @@ template <bool BUF>
 struct PbrLd {
   Ld<BUF, GL_STREAM_CP> codes, coords, offsets, data, rptr, rcol, rval;
   Ld<BUF> x;
+  Ld<BUF, GL_REM_X_CP> xr;  // X rows gathered by remainder entries
   __device__ __forceinline__ PbrLd(const struct PbrView& m, const RhsView& d);
 };
 
@@ -438,12 +447,13 @@ __device__ __forceinline__ PbrLd<BUF>::PbrLd(const PbrView& m, const RhsView& d)
       rptr(m.rem_ptr, (m.rows + 1) * 4),
       rcol(m.rem_col, m.num_rem * 4),
       rval(m.rem_val, m.num_rem * 4),
-      x(d.x + d.k0, (d.len - d.k0) * 4) {}
+      x(d.x + d.k0, (d.len - d.k0) * 4),
+      xr(d.x + d.k0, (d.len - d.k0) * 4) {}
 
 // N (1..8) consecutive floats of X starting at element i, zeros for lanes with valid <= 0.
 // `wide` (wave-uniform: 16-byte aligned rows and a full column chunk) selects float4 loads.
-template <int N, bool BUF>
-__device__ __forceinline__ void load_row(const Ld<BUF>& xl, int64_t i, bool wide, int valid,
+template <int N, bool BUF, int CP>
+__device__ __forceinline__ void load_row(const Ld<BUF, CP>& xl, int64_t i, bool wide, int valid,
                                          float (&out)[N]) {
   if constexpr (N >= 4) {
     if (wide) {
@@ -768,7 +778,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
         float xv[EBF][4];
 #pragma unroll
         for (int i = 0; i < EBF; ++i)
-          load_row<4>(ld.x, xrow<BUF>(fcol[i], d.ldx) + kq4, vec, e + i < ce ? nv4 : 0, xv[i]);
+          load_row<4>(ld.xr, xrow<BUF>(fcol[i], d.ldx) + kq4, vec, e + i < ce ? nv4 : 0, xv[i]);
         float av[EBF];
 #pragma unroll
         for (int i = 0; i < EBF; ++i) av[i] = fval[i];
@@ -830,7 +840,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
             const int c = __shfl(scol, gbase + ib + i, kWave);
             a[i] = __shfl(sval, gbase + ib + i, kWave);
             const bool ok = jb + ib + i < rs.n();
-            load_row<FPL>(ld.x, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
+            load_row<FPL>(ld.xr, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
           }
 #pragma unroll
           for (int i = 0; i < EB; ++i)
@@ -1087,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
           scol = (int)ld.rcol.u32(rs.j0 + jb + q, ok);
           sval = ld.rval.f32(rs.j0 + jb + q, ok);
         }
-        load_row<KW>(ld.x, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
+        load_row<KW>(ld.xr, xrow<BUF>(scol, d.ldx), vec, ok ? kwv : 0, xv);
 #pragma unroll
         for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(sval, xv[k], acc[k]);
       }
