@@ -64,7 +64,8 @@ __device__ __forceinline__ Sched xcd_sched(int64_t n) {
   const int x = blockIdx.x % kNumXcd;
   const int64_t i = blockIdx.x / kNumXcd;
   const int wpb = blockDim.x / kWave;
-  const int wid = threadIdx.x / kWave;
+  // the wave index is uniform across the wave: keep it (and the schedule) in scalar registers
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t lo = n * x / kNumXcd;
   const int64_t hi = n * (x + 1) / kNumXcd;
   Sched s;

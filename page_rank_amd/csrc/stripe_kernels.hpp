@@ -575,7 +575,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   if (!epi.active()) return;
   const PbrLd<BUF> ld(m, d);
   const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int g = lane / KW;
   const int f = lane % KW;
   const int o = lane & 15;
