@@ -6,6 +6,7 @@
 #include "stripe_kernels.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 // Matrix-core tile product for 32 / 64 columns of densely tiled 8x8 SpMMs (1 = on).
@@ -43,10 +44,10 @@ int resident_per_cu(F kernel) {
 // feature-lane remainders with KW / (64 / B) >= 4.
 #define GL_GO(KERN)                                                        \
   do {                                                                     \
-    static int occ_cached = -1;                                            \
+    static std::atomic<int> occ_cached{-1};                                \
     if (occ) {                                                             \
-      if (occ_cached < 0) occ_cached = resident_per_cu(KERN);              \
-      *occ = occ_cached;                                                   \
+      if (occ_cached.load() < 0) occ_cached.store(resident_per_cu(KERN));  \
+      *occ = occ_cached.load();                                            \
     } else {                                                               \
       hipLaunchKernelGGL(KERN, grid, block, 0, st, m, d, e);               \
     }                                                                      \
