@@ -21,7 +21,8 @@ import torch
 from .registry import get_pbr_gpu_meta
 
 KW_ALL = (1, 2, 4, 8, 16, 32, 64)
-GRID_MULTS = (4, 8, 16)
+# PPR grid: workgroups per CU; 0 = as many as the device holds of the step kernel at once
+GRID_MULTS = (0, 4, 8, 16)
 
 
 def kw_candidates(K: int):
