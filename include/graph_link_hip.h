@@ -125,7 +125,8 @@ typedef struct gl_ppr_opts {
   int32_t retire_mode;
   int32_t graph_iters;
   int32_t kw;          /* RHS columns per launch, 0 = library choice */
-  int32_t grid_mult;   /* workgroups per CU of the persistent launch, 0 = library choice */
+  int32_t grid_mult;   /* workgroups per CU of the persistent launch, 0 = as many as the
+                          device holds of the step kernel at once */
 } gl_ppr_opts;
 
 int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,
