@@ -78,6 +78,12 @@
 #ifndef GL_FEAT64_WAVES
 #define GL_FEAT64_WAVES 4
 #endif
+// Feature-lane kernels on 8x8 tiles with 32 or 64 columns: the DPP rows that share a tile
+// each decode a part of its values and exchange them with v_permlane16/32_swap (1), or every
+// row decodes all 64 (0).
+#ifndef GL_FEAT_SPLIT
+#define GL_FEAT_SPLIT 1
+#endif
 // Cache policy of the remainder's X-row gathers (0 = default, 2 = non-temporal, 16 / 17 =
 // device / system scope). Measured on MI355X: non-temporal 13 % slower at 32 columns, 2x for
 // the one-column remainder and 59 % for the 64-column PPR step (X is served from the
@@ -579,6 +585,20 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int g = lane / KW;
   const int f = lane % KW;
   const int o = lane & 15;
+  // Split decode: the R = 4 / G DPP rows of a tile each decode NL = NV / R of its 16-position
+  // chunks (row ri takes chunks ri * NL ..), and permlane swaps give every row all of them.
+  constexpr bool kSplit = GL_FEAT_SPLIT && B == 8 && G <= 2;
+  constexpr int R = kSplit ? 4 / G : 1;
+  constexpr int NL = NV / R;
+  const int ri = (lane >> 4) & (R - 1);
+  const bool whi = R == 2 ? ri == 1 : ri >= 2;  // this row's chunks lie in the code's high word
+  uint32_t sbit[NL], smask[NL];                 // bit of each decoded position, bits below it
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const int sh = (R == 2 ? 16 * j : 16 * (ri & 1)) + o;
+    sbit[j] = 1u << sh;
+    smask[j] = sbit[j] - 1u;
+  }
   const int64_t colk = d.k0 + f;
   const bool fok = colk < d.K;
   epi.lane_init(colk, fok);
@@ -625,11 +645,42 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       const uint32_t chi = __shfl(meta.chi, src, kWave);
       const int c0 = __shfl(meta.c0, src, kWave);
       const int off = __shfl(meta.off, src, kWave);
+      if constexpr (kSplit) {
+        const uint32_t w = whi ? chi : clo;
+        const int base = off + (whi ? (int)__popc(clo) : 0);
+        uint32_t dv[NL];
 #pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        const int p = 16 * j + o;
-        const bool has = tv && p < P && code_bit(clo, chi, p);
-        bt.v[u][j] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+        for (int j = 0; j < NL; ++j) {
+          const bool has = tv && (w & sbit[j]) != 0u;
+          dv[j] = ld.data.u32(base + (int)__popc(w & smask[j]), has);
+        }
+        // permlane16_swap(a, a) = (row 2k of a in rows 2k, 2k+1; row 2k+1 of a in both);
+        // permlane32_swap(a, a) = (rows 0-1 of a in both halves; rows 2-3 in both)
+        if constexpr (R == 2) {
+#pragma unroll
+          for (int j = 0; j < NL; ++j) {
+            const auto pr = __builtin_amdgcn_permlane16_swap(dv[j], dv[j], false, false);
+            bt.v[u][j] = __builtin_bit_cast(float, (uint32_t)pr[0]);
+            bt.v[u][NL + j] = __builtin_bit_cast(float, (uint32_t)pr[1]);
+          }
+        } else {
+          const auto p16 = __builtin_amdgcn_permlane16_swap(dv[0], dv[0], false, false);
+          const auto qa = __builtin_amdgcn_permlane32_swap((uint32_t)p16[0], (uint32_t)p16[0],
+                                                           false, false);
+          const auto qb = __builtin_amdgcn_permlane32_swap((uint32_t)p16[1], (uint32_t)p16[1],
+                                                           false, false);
+          bt.v[u][0] = __builtin_bit_cast(float, (uint32_t)qa[0]);
+          bt.v[u][1] = __builtin_bit_cast(float, (uint32_t)qb[0]);
+          bt.v[u][2] = __builtin_bit_cast(float, (uint32_t)qa[1]);
+          bt.v[u][3] = __builtin_bit_cast(float, (uint32_t)qb[1]);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          const int p = 16 * j + o;
+          const bool has = tv && p < P && code_bit(clo, chi, p);
+          bt.v[u][j] = ld.data.f32(off + code_prefix(clo, chi, p), has);
+        }
       }
       const int64_t xb = xrow<BUF>(c0, d.ldx) + f;
 #pragma unroll
