@@ -65,10 +65,11 @@
 #ifndef GL_REM_EB
 #define GL_REM_EB 4
 #endif
-// The same for the 64-column feature-lane kernel (measured: 2 is 5 % faster there, 4 is
-// 12 % faster at 32 columns).
+// The same for the 64-column feature-lane kernel. Measured: 1 entry keeps the PPR step at
+// 74 VGPRs (6 waves per SIMD) and is 1 % faster than 2 there; 2 was 5 % faster than 4 with
+// the registers of an earlier form of the kernel; 4 is 12 % faster at 32 columns.
 #ifndef GL_REM_EB64
-#define GL_REM_EB64 2
+#define GL_REM_EB64 1
 #endif
 // Minimum waves per SIMD the feature-lane kernel is compiled for (caps its VGPRs); 0 = free.
 #ifndef GL_FEAT_WAVES
