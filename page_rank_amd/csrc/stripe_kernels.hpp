@@ -71,6 +71,11 @@
 #ifndef GL_REM_EB64
 #define GL_REM_EB64 1
 #endif
+// The same for the plain-pointer form (arrays of 2 GB or more): 1 entry measured 15 % slower
+// than 2 for the 16M-node PPR step.
+#ifndef GL_REM_EB64_PTR
+#define GL_REM_EB64_PTR 2
+#endif
 // Minimum waves per SIMD the feature-lane kernel is compiled for (caps its VGPRs); 0 = free.
 #ifndef GL_FEAT_WAVES
 #define GL_FEAT_WAVES 0
@@ -568,7 +573,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int FPL = (KW / L) > 0 ? (KW / L) : 1;  // remainder: RHS columns per lane
   // remainder: X gathers issued together (fewer at 64 columns, where each holds 8 floats
   // per lane and registers decide the occupancy)
-  constexpr int EBW = KW >= 64 ? GL_REM_EB64 : GL_REM_EB;
+  constexpr int EBW = KW >= 64 ? (BUF ? GL_REM_EB64 : GL_REM_EB64_PTR) : GL_REM_EB;
   constexpr int EB = L < EBW ? L : EBW;
   static_assert(L % EB == 0, "remainder batches must not straddle a block of L entries");
   // flat remainder: LE lanes (4 columns each) per entry, NGR entry groups per wave
