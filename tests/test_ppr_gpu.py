@@ -186,3 +186,48 @@ def test_more_seed_sets_than_one_launch_width(K, device):
     assert np.all(np.abs(res.iterations - it_o) <= 1)
     assert np.abs(X - Xo).sum(axis=0).max() < 2e-6
     assert np.allclose(X.sum(axis=0), 1.0, atol=1e-5)
+
+
+def test_plain_pointer_step(device):
+    """A graph of more than 2^24 nodes takes the plain-pointer form of the fused step (64-bit
+    offsets; X is 4.3 GB at 64 columns). 400 groups of 8 consecutive node ids, spread over
+    the whole id range, carry every edge: dense 8x8 tiles inside a group and 3 random edges
+    per node to other groups; every other node keeps 0. 64 seed sets against the FP64
+    oracle on the active nodes, with the same stopping rule."""
+    rng = np.random.default_rng(9)
+    n, groups = (1 << 24) + 16, 400
+    bases = rng.choice((n - 8) // 8, groups, replace=False) * 8
+    nodes = (bases[:, None] + np.arange(8)[None, :]).ravel()
+    na = nodes.size
+    src, dst = [], []
+    for g in range(groups):
+        blk = rng.random((8, 8)) < 0.6
+        a, b = np.nonzero(blk)
+        src.append(g * 8 + a)
+        dst.append(g * 8 + b)
+    src.append(np.repeat(np.arange(na), 3))
+    dst.append(rng.integers(0, na, 3 * na))
+    adj = sp.csr_matrix((np.ones(sum(map(len, src))), (np.concatenate(src), np.concatenate(dst))),
+                        shape=(na, na))
+    adj.sum_duplicates()
+    adj.data[:] = 1.0
+    A_loc, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    assert not dangling.any()
+    A32 = sp.csr_matrix(A_loc, dtype=np.float32).tocoo()
+    A_glob = sp.csr_matrix((A32.data, (nodes[A32.row], nodes[A32.col])), shape=(n, n))
+    pbr = graph_link.csr_to_pbr(A_glob, 8, 8, 4).to(device)
+    K = 64
+    sets_loc = [rng.choice(na, size=int(rng.integers(1, 20)), replace=False) for _ in range(K)]
+    res = graph_link.run_ppr(pbr, [nodes[s] for s in sets_loc], alpha=0.85, tol=1e-6,
+                             max_iter=200, retire="per_column", graph_iters=4)
+    P = oppr.personalization_matrix(na, sets_loc)
+    Xo, it_o, conv_o, _ = oppr.power(sp.csr_matrix(A32), P, 0.85, 1e-6, 200,
+                                     retire="per_column")
+    idx = torch.from_numpy(nodes).to(device)
+    X = res.X[idx].cpu().numpy()
+    assert res.converged.all() and conv_o.all()
+    assert np.all(np.abs(res.iterations - it_o) <= 1)
+    assert np.abs(X - Xo).sum(axis=0).max() < 2e-6
+    # every other node stays exactly 0
+    nz_rows = torch.nonzero((res.X != 0).any(dim=1)).flatten().cpu().numpy()
+    assert np.isin(nz_rows, nodes).all()

@@ -206,3 +206,37 @@ def test_spmm_densely_tiled_8x8(kw, mode, device):
         out = ys.cpu().numpy()
         _check(A, X, out[:, :K] - 2.0)
         assert np.all(out[:, K:] == 2.0)
+
+
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_plain_pointer_kernels(mode, device):
+    """A matrix with more than 2^24 columns takes the plain-pointer form of the stripe
+    kernels (64-bit element offsets; the buffer-descriptor form needs columns and the X row
+    pitch below 2^24 and every array below 2 GB). Densely tiled 8x8 stripes with their
+    columns spread over the whole range, so the whole-stripe batches run, plus a scattered
+    remainder; K covers the bit-lane and both wide feature-lane kernels (X reaches 4.3 GB)."""
+    rng = np.random.default_rng(5)
+    n, cols = 8 * 40 + 5, (1 << 24) + 40
+    rows_l, cols_l = [], []
+    for s in range((n + 7) // 8):
+        for bc in rng.choice(cols // 8, 9, replace=False):
+            pos = rng.choice(64, int(rng.integers(4, 41)), replace=False)
+            rows_l.append(s * 8 + pos // 8)
+            cols_l.append(bc * 8 + pos % 8)
+    rows_l.append(rng.integers(0, n, 600))
+    cols_l.append(rng.integers(0, cols, 600))
+    r, c = np.concatenate(rows_l), np.concatenate(cols_l)
+    keep = r < n
+    A = sp.csr_matrix((rng.random(int(keep.sum())).astype(np.float32) + 0.1,
+                       (r[keep], c[keep])), shape=(n, cols))
+    A.sum_duplicates()
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
+    assert pbr.accounted_blocks() >= 6 * ((n + 7) // 8)
+    used = np.unique(A.indices)
+    A_used = A[:, used]
+    idx = torch.from_numpy(used).to(device)
+    for K in (1, 8, 32, 64):
+        X = torch.rand((cols, K), device=device)
+        Y = graph_link.pbr_matmul(pbr, X, mode=mode).cpu().numpy()
+        _check(A_used, X[idx].cpu().numpy(), Y)
+        del X
