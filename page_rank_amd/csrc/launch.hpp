@@ -17,6 +17,13 @@
 #ifndef GL_MFMA
 #define GL_MFMA 0
 #endif
+// VEC kernels for buffer-descriptor launches: 0 = full, 16-byte aligned column chunks only;
+// 1 = partial chunks too; 2 = any row pitch as well (dword-aligned vector loads). Measured on
+// MI355X (1M x 1M, 8x8) against 0: K=3 198 -> 125 us, K=6 306 -> 164 us, K=20 249 -> 207 us,
+// K=24 251 -> 218 us, K=48 487 -> 345 us; full aligned chunks unchanged.
+#ifndef GL_VEC_PARTIAL
+#define GL_VEC_PARTIAL 2
+#endif
 
 namespace gl {
 
@@ -85,8 +92,15 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
       return GL_OK;
     }
   }
-  // float4 X rows and a full column chunk: the VEC kernels (no runtime branch per X load)
-  const bool vec = (d.ldx & 3) == 0 && std::min<int64_t>(kw, d.K - d.k0) == kw;
+  // float4 X-row loads (the VEC kernels, no runtime branch per X load). The plain-pointer
+  // form needs 16-byte aligned rows and a full column chunk. Buffer loads need only dword
+  // alignment and are range-checked per dword, so there a partial last chunk reads the
+  // columns past K as well: the next row's values (0 past the array), which only reach
+  // accumulators of columns that are not stored.
+  const bool full = std::min<int64_t>(kw, d.K - d.k0) == kw;
+  const bool vec = GL_VEC_PARTIAL == 2 && BUF
+                       ? true
+                       : (d.ldx & 3) == 0 && (full || (GL_VEC_PARTIAL >= 1 && BUF));
   switch (kw) {
 #define GL_KB(KW)                                                                        \
     case KW:                                                                             \
