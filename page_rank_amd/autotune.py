@@ -30,6 +30,8 @@ def kw_candidates(K: int):
     the smallest one that covers K in one launch (and at least one feature-lane width)."""
     top = next((w for w in KW_ALL if w >= K), 64)
     out = [w for w in KW_ALL if w <= top]
+    if 1 < top < 4:
+        out.append(4)  # two columns can run faster in the 4-column kernel (float4 rows)
     if top < 16:
         out.append(16)
     return out[-4:] if len(out) > 4 else out
