@@ -186,10 +186,11 @@ int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, 0,
             codes, coords, offsets, data, stripe_ptr, nullptr, nullptr, nullptr};
   const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
-  for (int64_t k0 = 0; k0 < K; k0 += kw) {
+  for (int64_t k0 = 0, w = kw; k0 < K; k0 += w) {
+    w = chunk_kw(kw, K - k0);
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
-    if ((rc = launch_stripe<EpiStore, true, false>(m, d, block_rows, kw, e, nwg,
+    if ((rc = launch_stripe<EpiStore, true, false>(m, d, block_rows, (int)w, e, nwg,
                                                    (hipStream_t)stream)))
       return rc;
   }
@@ -211,10 +212,12 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* 
   PbrView m{rows, cols, (rows + 7) / 8, 0, 0, nnz, nullptr, nullptr, nullptr, nullptr, nullptr,
             indptr, col_ind, vals};
   const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
-  for (int64_t k0 = 0; k0 < K; k0 += kw) {
+  for (int64_t k0 = 0, w = kw; k0 < K; k0 += w) {
+    w = chunk_kw(kw, K - k0);
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
-    if ((rc = launch_stripe<EpiStore, false, true>(m, d, 8, kw, e, nwg, (hipStream_t)stream)))
+    if ((rc = launch_stripe<EpiStore, false, true>(m, d, 8, (int)w, e, nwg,
+                                                   (hipStream_t)stream)))
       return rc;
   }
   return GL_OK;
@@ -258,10 +261,12 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, num_rem,
             codes, coords, offsets, data, stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
   const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
-  for (int64_t k0 = 0; k0 < K; k0 += kw) {
+  for (int64_t k0 = 0, w = kw; k0 < K; k0 += w) {
+    w = chunk_kw(kw, K - k0);
     RhsView d{X, ldx, K, k0, cols * ldx};
     EpiStore e{Y, ldy, accumulate, fits_u24(rows, ldy)};
-    if ((rc = launch_stripe<EpiStore, true, true>(m, d, block_rows, kw, e, nwg, st))) return rc;
+    if ((rc = launch_stripe<EpiStore, true, true>(m, d, block_rows, (int)w, e, nwg, st)))
+      return rc;
   }
   return GL_OK;
 }

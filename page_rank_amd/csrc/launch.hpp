@@ -33,6 +33,14 @@ hipEvent_t lib_event(int i);
 int grid_for(int64_t work_items, int wg_per_cu);
 int choose_kw(int64_t K);
 bool valid_kw(int kw);
+// Width of the launch for columns [k0, K) taken kw at a time: the last, partial chunk runs
+// in the narrowest kernel that covers it (a 64-column launch with 32 live columns costs
+// about 1.5x a 32-column one).
+inline int chunk_kw(int kw, int64_t left) {
+  if (left >= kw) return kw;
+  const int c = choose_kw(left);
+  return c < kw ? c : kw;
+}
 
 // One launch of the bit-lane (FEAT false) or feature-lane kernel. The VEC form exists only
 // where a lane's slice of an X row is 4 or more floats: bit-lane kernels of 4 and 8 columns,

@@ -386,13 +386,14 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   auto one_iteration = [&](int parity) -> int {
     const float* xc = bufs[parity];
     float* xn = bufs[parity ^ 1];
-    for (int64_t k0 = 0; k0 < K; k0 += kw) {
+    for (int64_t k0 = 0, cw = kw; k0 < K; k0 += cw) {
+      cw = chunk_kw(kw, K - k0);
       RhsView d{xc, K, K, k0, rows * K};
       EpiPpr e{xc, xn, K, opts->alpha, w.cu, w.tele, w.frozen,
                opts->dangling_mode != GL_DANGLING_NONE ? dangling_mask : nullptr,
                w.sptr, w.srow, w.scol, w.sw, w.ctl, w.part_res, w.part_dang, K, k0,
                fits_u24(rows, K), rows * K, 0.f, 0.f, 0, 0};
-      int rc = launch_stripe<EpiPpr, true, true>(m, d, block_rows, kw, e, nwg, st);
+      int rc = launch_stripe<EpiPpr, true, true>(m, d, block_rows, (int)cw, e, nwg, st);
       if (rc) return rc;
     }
     hipLaunchKernelGGL(k_retire, dim3(1), dim3(kRetireThreads), 0, st, w.ctl, cs, K, nwg,
