@@ -554,8 +554,9 @@ __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const Pbr
 // --------------------------------------------------------------------------------------
 
 // Feature-lane kernel, KW in {16, 32, 64}. UF: tile groups whose loads are issued together
-// (the launcher picks 4 for 8x8 matrices with about 8 tiles per stripe, else 1). VEC: X rows
-// are float4-aligned and the launch's column chunk is full.
+// (the launcher picks a whole stripe, 2 groups at 16 columns and 4 at 32 and 64, for 8x8
+// matrices with about 8 tiles per stripe, else 1). VEC: X rows load as float4 (buffer
+// launches always; plain-pointer launches for aligned rows and a full column chunk).
 template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UF, bool VEC>
 __global__ void __launch_bounds__(256)
 #if GL_FEAT_WAVES > 0
