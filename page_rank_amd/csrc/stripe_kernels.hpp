@@ -319,7 +319,7 @@ struct EpiStore {
     else if (GL_NT_STORE) __builtin_nontemporal_store(acc, p);
     else *p = acc;
   }
-  __device__ __forceinline__ void finish(int, float*) {}
+  __device__ __forceinline__ void finish(int) {}
 };
 
 // Device control block of a PPR run.
@@ -408,20 +408,36 @@ struct EpiPpr {
     if (GL_NT_STORE) __builtin_nontemporal_store(xn, xnext + ix);
     else xnext[ix] = xn;
   }
-  // slot: the lane's column minus k0 (or -1); red: 2*64 floats of shared memory.
-  __device__ __forceinline__ void finish(int slot, float* red) {
-    if (threadIdx.x < 128) red[threadIdx.x] = 0.f;
+  // slot: the lane's column minus k0 (or -1). The workgroup's partial of a column is the
+  // sum of its lanes' values taken in thread order, so partials (and the residuals built
+  // from them) do not depend on which wave finishes first. Once per workgroup and launch.
+  __device__ __forceinline__ void finish(int slot) {
+    __shared__ float s_res[256], s_dang[256];
+    __shared__ int s_slot[256];
+    __shared__ float s_q[2][4][64];
+    const int t = threadIdx.x;
+    s_res[t] = res;
+    s_dang[t] = dang;
+    s_slot[t] = slot;
     __syncthreads();
-    if (slot >= 0) {
-      atomicAdd(&red[slot], res);
-      atomicAdd(&red[64 + slot], dang);
+    {  // thread (quarter q, slot c) sums the lanes of wave q whose slot is c
+      const int c = t & 63, q = t >> 6;
+      float r = 0.f, dg = 0.f;
+      for (int i = 64 * q; i < 64 * q + 64; ++i) {
+        if (s_slot[i] == c) {
+          r += s_res[i];
+          dg += s_dang[i];
+        }
+      }
+      s_q[0][q][c] = r;
+      s_q[1][q][c] = dg;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-      const int64_t col = k0 + threadIdx.x;
+    if (t < 64) {
+      const int64_t col = k0 + t;
       if (col < K) {
-        part_res[blockIdx.x * K + col] = red[threadIdx.x];
-        part_dang[blockIdx.x * K + col] = red[64 + threadIdx.x];
+        part_res[blockIdx.x * K + col] = ((s_q[0][0][t] + s_q[0][1][t]) + s_q[0][2][t]) + s_q[0][3][t];
+        part_dang[blockIdx.x * K + col] = ((s_q[1][0][t] + s_q[1][1][t]) + s_q[1][2][t]) + s_q[1][3][t];
       }
     }
   }
@@ -581,7 +597,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int LE = KW / 4;
   constexpr int NGR = kWave / LE;
   constexpr int EBF = GL_REM_FLAT_EB;
-  __shared__ float red[128];
   __shared__ float racc[4][B * KW];
   __shared__ int rsb_all[4][B + 1];
 
@@ -951,7 +966,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     rs = rsn;
 #endif
   }
-  epi.finish(fok ? f : -1, red);
+  epi.finish(fok ? f : -1);
 }
 
 // --------------------------------------------------------------------------------------
@@ -968,7 +983,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   constexpr int L = kWave / B;      // lanes per matrix row
   // densely tiled 8x8 matrices (UB == 8, whole-stripe batches): row lanes (GL_BIT_ROWLANE)
   constexpr bool ROWLANE = GL_BIT_ROWLANE && B == 8 && UB == 8 && KW == 1;
-  __shared__ float red[128];
 
   if (!epi.active()) return;
   const PbrLd<BUF> ld(m, d);
@@ -1191,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     mt = mtn;
     rs = rsn;
   }
-  epi.finish(q < kwv ? q : -1, red);
+  epi.finish(q < kwv ? q : -1);
 }
 
 }  // namespace gl

@@ -231,3 +231,21 @@ def test_plain_pointer_step(device):
     # every other node stays exactly 0
     nz_rows = torch.nonzero((res.X != 0).any(dim=1)).flatten().cpu().numpy()
     assert np.isin(nz_rows, nodes).all()
+
+
+def test_repeat_runs_are_bitwise_identical(device):
+    """The step owns its output rows and the retire kernel sums the per-workgroup partials
+    in a fixed order, so two runs give the same iterates, residual history and iteration
+    counts, bit for bit."""
+    n = 900
+    adj = _sbm(n, 90, 0.08, 0.004, seed=4)
+    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    pbr = graph_link.csr_to_pbr(sp.csr_matrix(A, dtype=np.float32), 8, 8, 2).to(device)
+    rng = np.random.default_rng(4)
+    sets = [rng.choice(n, size=int(rng.integers(1, 12)), replace=False) for _ in range(40)]
+    runs = [graph_link.run_ppr(pbr, sets, tol=1e-6, max_iter=100, dangling=dangling,
+                               dangling_mode="uniform") for _ in range(2)]
+    a, b = runs
+    assert torch.equal(a.X, b.X)
+    assert np.array_equal(a.iterations, b.iterations)
+    assert np.array_equal(np.asarray(a.history), np.asarray(b.history))

@@ -240,3 +240,16 @@ def test_plain_pointer_kernels(mode, device):
         Y = graph_link.pbr_matmul(pbr, X, mode=mode).cpu().numpy()
         _check(A_used, X[idx].cpu().numpy(), Y)
         del X
+
+
+@pytest.mark.parametrize("K", [1, 8, 32, 64])
+def test_repeat_runs_are_bitwise_identical(K, device):
+    """Every output element is owned by one wave and summed in a fixed order (no atomics),
+    so repeated products are bitwise identical whatever order the workgroups run in."""
+    n = 8 * 300 + 5
+    A = blocky_csr(n, 8, 10, 16, 64, scatter=3000, seed=21, band=40)
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4).to(device)
+    X = torch.from_numpy(random_x(n, K, seed=K)).to(device)
+    ref = graph_link.pbr_matmul(pbr, X)
+    for _ in range(3):
+        assert torch.equal(graph_link.pbr_matmul(pbr, X), ref)
