@@ -46,21 +46,49 @@ __global__ void k_seed_stripes(int32_t* __restrict__ sptr, int64_t num_stripes, 
   }
 }
 
-// Dangling mass of the start vector: dsum[k] = sum_{i: dmask[i]} X[i, k].
-__global__ void k_dangling_sum(const float* __restrict__ x, int64_t ld, int64_t N, int64_t K,
-                               const uint8_t* __restrict__ dmask, float* __restrict__ dsum) {
-  __shared__ float acc[kMaxK];
-  for (int k = threadIdx.x; k < K; k += blockDim.x) acc[k] = 0.f;
-  __syncthreads();
-  const int64_t total = N * K;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t row = i / K;
-    if (dmask[row]) atomicAdd(&acc[i % K], x[row * ld + (i % K)]);
+// Dangling mass of the start vector, dsum[k] = sum_{i: dmask[i]} X[i, k], in a fixed order:
+// block b sums rows [b * rpb, (b + 1) * rpb) (thread t: column k0 + t % W, every RP-th row
+// from t / W), combines its threads in thread order, and k_dangling_final adds the block
+// partials in block order. W = min(K, 256) columns per pass, RP = 256 / W.
+constexpr int kDangBlocks = 1024;
+static_assert(kDangBlocks <= kMaxWg, "dangling partials reuse the step's partial rows");
+
+__global__ void __launch_bounds__(256) k_dangling_partial(const float* __restrict__ x,
+                                                          int64_t N, int64_t K,
+                                                          const uint8_t* __restrict__ dmask,
+                                                          int64_t rpb, float* __restrict__ part) {
+  __shared__ float red[256];
+  const int W = (int)(K < 256 ? K : 256);
+  const int RP = 256 / W;
+  const int t = threadIdx.x;
+  const int lc = t % W, lr = t / W;
+  const int64_t r0 = blockIdx.x * rpb;
+  const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
+  for (int64_t k0 = 0; k0 < K; k0 += W) {
+    const int64_t k = k0 + lc;
+    float acc = 0.f;
+    if (lr < RP && k < K)
+      for (int64_t i = r0 + lr; i < r1; i += RP)
+        if (dmask[i]) acc += x[i * K + k];
+    red[t] = acc;
+    __syncthreads();
+    if (t < W && k0 + t < K) {
+      float sum = 0.f;
+      for (int r = 0; r < RP; ++r) sum += red[r * W + t];
+      part[blockIdx.x * K + k0 + t] = sum;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < K; k += blockDim.x)
-    if (acc[k] != 0.f) atomicAdd(&dsum[k], acc[k]);
+}
+
+__global__ void k_dangling_final(const float* __restrict__ part, int64_t nb, int64_t K,
+                                 float* __restrict__ dsum) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int64_t b = 0; b < nb; ++b) s += part[b * K + k];
+    dsum[k] = s;
+  }
 }
 
 struct ColState {
@@ -353,9 +381,15 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   }
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(K, 256)), dim3(256), 0, st, w.dsum, K, 0.f);
   if (opts->dangling_mode != GL_DANGLING_NONE) {
-    hipLaunchKernelGGL(k_dangling_sum, dim3(std::min<int64_t>(blocks_for(NK, 256), 2048)),
-                       dim3(256), 0, st, X_out, K, rows, K, dangling_mask, w.dsum);
-    GL_LAUNCH_CHECK("k_dangling_sum");
+    // block partials in w.part_res (kMaxWg >= kDangBlocks rows of K), unused until the loop
+    const int64_t rpb = (rows + kDangBlocks - 1) / kDangBlocks;
+    const int64_t nb = (rows + rpb - 1) / rpb;
+    hipLaunchKernelGGL(k_dangling_partial, dim3((unsigned)nb), dim3(256), 0, st, X_out, rows, K,
+                       dangling_mask, rpb, w.part_res);
+    GL_LAUNCH_CHECK("k_dangling_partial");
+    hipLaunchKernelGGL(k_dangling_final, dim3(blocks_for(K, 256)), dim3(256), 0, st,
+                       (const float*)w.part_res, nb, K, w.dsum);
+    GL_LAUNCH_CHECK("k_dangling_final");
   }
   ColState cs{w.frozen, iterations, converged, w.dsum, w.cu, w.tele};
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1024), 0, st, w.ctl, cs, K, opts->alpha, rows,

@@ -238,14 +238,20 @@ def test_repeat_runs_are_bitwise_identical(device):
     in a fixed order, so two runs give the same iterates, residual history and iteration
     counts, bit for bit."""
     n = 900
-    adj = _sbm(n, 90, 0.08, 0.004, seed=4)
-    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    adj = _sbm(n, 90, 0.08, 0.004, seed=4).tolil()
+    adj[::7, :] = 0                       # 129 dangling nodes
+    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.tocsr().T))
+    assert dangling.sum() >= 100
     pbr = graph_link.csr_to_pbr(sp.csr_matrix(A, dtype=np.float32), 8, 8, 2).to(device)
     rng = np.random.default_rng(4)
     sets = [rng.choice(n, size=int(rng.integers(1, 12)), replace=False) for _ in range(40)]
-    runs = [graph_link.run_ppr(pbr, sets, tol=1e-6, max_iter=100, dangling=dangling,
-                               dangling_mode="uniform") for _ in range(2)]
-    a, b = runs
-    assert torch.equal(a.X, b.X)
-    assert np.array_equal(a.iterations, b.iterations)
-    assert np.array_equal(np.asarray(a.history), np.asarray(b.history))
+    # a start vector with mass on every node, so the first dangling sum has many terms
+    x0 = rng.random((n, 40)).astype(np.float32)
+    x0 /= x0.sum(axis=0, keepdims=True)
+    for mode in ("uniform", "personalization"):
+        runs = [graph_link.run_ppr(pbr, sets, tol=1e-6, max_iter=100, dangling=dangling,
+                                   dangling_mode=mode, x0=x0) for _ in range(2)]
+        a, b = runs
+        assert torch.equal(a.X, b.X)
+        assert np.array_equal(a.iterations, b.iterations)
+        assert np.array_equal(np.asarray(a.history), np.asarray(b.history))
