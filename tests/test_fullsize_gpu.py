@@ -3,8 +3,9 @@
 SpMM on the 1M x 1M block-banded matrix (8 tiles per block row within +-512 blocks, 5M
 scattered remainder entries, ~27M nnz) at K = 1, 8, 32: sampled rows against the FP64 scipy
 product, the column checksum 1^T (A X) = (1^T A) X in FP64, and linearity. Batched PPR on the
-4M-node / 64M-edge block-clustered graph with 64 seed sets: every column a probability
-vector and a fixed point of the PPR map to the stopping tolerance.
+4M-node / 64M-edge block-clustered graph with 64 seed sets, and on the 16M-node / 256M-edge
+one with 128: every column a probability vector and a fixed point of the PPR map to the
+stopping tolerance.
 """
 import numpy as np
 import pytest
@@ -75,4 +76,32 @@ def test_ppr_4m_fixed_point_and_mass(device):
     resid = R.abs().double().sum(dim=0).cpu().numpy()
     # the run stops once an update is below tol; the next update is alpha times smaller in
     # norm at most, plus FP32 rounding of the 4M-term sums
+    assert resid.max() < 2 * tol
+
+
+def test_ppr_16m_fixed_point_and_mass(device):
+    """The full-scale run: 16M nodes, 256M edges, 16,000 communities, 128 seed sets of
+    U{1..200}. X is 8.2 GB, so the step takes its plain-pointer form."""
+    n, m, K = 16_000_000, 256_000_000, 128
+    src, dst = graphs.clustered_edges(n, m, communities=n // 1000, seed=3)
+    A, dang = graphs.transition_matrix(src, dst, n)
+    del src, dst
+    assert not dang.any()
+    pbr = graph_link.csr_to_pbr(A, 8, 8, 4, order="canonical").to(device)
+    del A
+    sets = graphs.seed_sets(n, K, 1, 200, seed=4)
+    alpha, tol = 0.85, 1e-6
+    res = graph_link.run_ppr(pbr, sets, alpha=alpha, tol=tol, max_iter=300,
+                             retire="per_column", return_history=False)
+    X = res.X
+    assert res.converged.all()
+    assert (X >= 0).all().item()
+    mass = torch.sum(X, dim=0, dtype=torch.float64).cpu().numpy()
+    assert np.allclose(mass, 1.0, atol=1e-4)
+    R = graph_link.pbr_matmul(pbr, X)
+    R.mul_(alpha).sub_(X)
+    for k, s in enumerate(sets):
+        idx = torch.as_tensor(np.asarray(s, dtype=np.int64), device=device)
+        R[idx, k] += (1.0 - alpha) / len(s)
+    resid = torch.sum(R.abs_(), dim=0, dtype=torch.float64).cpu().numpy()
     assert resid.max() < 2 * tol
