@@ -83,7 +83,9 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
   const int lane = threadIdx.x & (kWave - 1);
   const int r = lane % GL;
   const int gidx = lane / GL;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  // uniform across the wave: scalar registers
+  const int64_t wave = __builtin_amdgcn_readfirstlane(
+      (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
   const V* __restrict__ vals = (const V*)o.values;
 
@@ -197,14 +199,18 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
 // ---------------------------------------------------------------------------------------
 constexpr int kHashCap = 256;
 
+// FILL: the fill pass's arrays (value offsets, sorted keys) exist only there, so the count
+// pass needs less LDS per wave and holds more waves.
+template <bool FILL>
 struct HashLds {
+  static constexpr int kF = FILL ? kHashCap : 1;
   int32_t key[kHashCap];
   uint32_t lo[kHashCap], hi[kHashCap];
   int32_t rmin[kHashCap];
   int32_t tile[kHashCap];  // per slot: tile index (fill) or 1 (count) when kept, -1 if not
-  int32_t voff[kHashCap];  // per slot: offset of the tile's first value
-  int32_t skey[kHashCap];  // sorted keys / their slots
-  int32_t sslot[kHashCap];
+  int32_t voff[kF];        // per slot: offset of the tile's first value
+  int32_t skey[kF];        // sorted keys / their slots
+  int32_t sslot[kF];
   int32_t rows[65];        // CSR start of each row of the stripe (block_rows + 1 values)
   int32_t rcnt[64];        // rejected entries per row so far
   int32_t flag;            // table overflow
@@ -231,10 +237,12 @@ __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
 
 template <bool FILL, typename V>
 __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
-  __shared__ HashLds sh_all[4];
+  __shared__ HashLds<FILL> sh_all[4];
   const int lane = threadIdx.x & (kWave - 1);
-  HashLds& h = sh_all[threadIdx.x / kWave];
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave;
+  HashLds<FILL>& h = sh_all[threadIdx.x / kWave];
+  // uniform across the wave: scalar registers
+  const int64_t wave = __builtin_amdgcn_readfirstlane(
+      (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
   const V* __restrict__ vals = (const V*)o.values;
   const int br = a.br, bc = a.bc;
@@ -440,15 +448,17 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
       const int ls = valid ? max(0, h.rows[r] - c) : 0;
       const uint64_t from_ls = ls == 0 ? ~0ull : ~(~0ull >> (64 - ls));
       const int32_t before = __popcll(rm & below_lane & from_ls);
-      if (FILL && valid) {
-        if (rej) {
-          const int32_t pos = o.nr[r0 + r] + h.rcnt[r] + before;
-          o.rem_col[pos] = col;
-          ((V*)o.rem_val)[pos] = vals[e];
-        } else {
-          const uint64_t code = ((uint64_t)h.hi[slot] << 32) | h.lo[slot];
-          const uint64_t below = bit == 0 ? 0ull : (~0ull >> (64 - bit));
-          ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = vals[e];
+      if constexpr (FILL) {
+        if (valid) {
+          if (rej) {
+            const int32_t pos = o.nr[r0 + r] + h.rcnt[r] + before;
+            o.rem_col[pos] = col;
+            ((V*)o.rem_val)[pos] = vals[e];
+          } else {
+            const uint64_t code = ((uint64_t)h.hi[slot] << 32) | h.lo[slot];
+            const uint64_t below = bit == 0 ? 0ull : (~0ull >> (64 - bit));
+            ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = vals[e];
+          }
         }
       }
       // the last lane of each row in this chunk advances that row's count
@@ -594,7 +604,8 @@ static int group_lanes(int br) {
 template <bool FILL, typename V>
 static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
   if (o.big != nullptr) {
-    const int64_t cap = (int64_t)device_info().num_cus * 16;  // waves
+    // waves: what a CU holds of each pass (LDS: 4 workgroups for fill, 7 for count)
+    const int64_t cap = (int64_t)device_info().num_cus * (FILL ? 16 : 28);
     const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
     hipLaunchKernelGGL((k_build_hash<FILL, V>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
                        st, a, o);
