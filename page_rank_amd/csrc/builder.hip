@@ -16,6 +16,7 @@
 // list every distinct block (block column, first row, kept tile or -1) so that the host can
 // replay the reference's container operations and gl_pbr_permute_tiles can reorder.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <unordered_map>
 #include <vector>
@@ -198,19 +199,24 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
 // distinct blocks are flagged and left to the merge kernel.
 // ---------------------------------------------------------------------------------------
 constexpr int kHashCap = 256;
+// Register budget of the fill pass: 6 waves per SIMD (80 VGPRs), which is also what its LDS
+// allows (6 workgroups of 4 waves per CU).
+constexpr int kHashWavesPerSimd = 6;
+// Grid of the count pass, in workgroups per CU: 7 of the 8 its LDS allows (0.71 ms for the
+// whole 1M x 1M build, against 0.77 ms with 8 and with 4).
+constexpr int kHashCountWgPerCu = 7;
 
-// FILL: the fill pass's arrays (value offsets, sorted keys) exist only there, so the count
-// pass needs less LDS per wave and holds more waves.
+// FILL: the fill pass's arrays (value offsets, slots in key order) exist only there, so the
+// count pass needs less LDS per wave and holds more waves. A block's first row is not kept:
+// it is the lowest set bit of its code divided by block_cols.
 template <bool FILL>
 struct HashLds {
   static constexpr int kF = FILL ? kHashCap : 1;
   int32_t key[kHashCap];
   uint32_t lo[kHashCap], hi[kHashCap];
-  int32_t rmin[kHashCap];
   int32_t tile[kHashCap];  // per slot: tile index (fill) or 1 (count) when kept, -1 if not
   int32_t voff[kF];        // per slot: offset of the tile's first value
-  int32_t skey[kF];        // sorted keys / their slots
-  int32_t sslot[kF];
+  int32_t sslot[kF];       // slots in ascending key order
   int32_t rows[65];        // CSR start of each row of the stripe (block_rows + 1 values)
   int32_t rcnt[64];        // rejected entries per row so far
   int32_t flag;            // table overflow
@@ -236,7 +242,8 @@ __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
 }
 
 template <bool FILL, typename V>
-__global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
+__global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
+    k_build_hash(BuildArgs a, FillOut o) {
   __shared__ HashLds<FILL> sh_all[4];
   const int lane = threadIdx.x & (kWave - 1);
   HashLds<FILL>& h = sh_all[threadIdx.x / kWave];
@@ -256,7 +263,6 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
       h.key[i] = -1;
       h.lo[i] = 0u;
       h.hi[i] = 0u;
-      h.rmin[i] = INT32_MAX;
     }
     if (lane == 0) h.flag = 0;
     wsync();
@@ -300,7 +306,6 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
         } else {
           if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
           else atomicOr(&h.hi[slot], 1u << (bit - 32));
-          atomicMin(&h.rmin[slot], r);
         }
       }
     }
@@ -337,7 +342,8 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
         o.nd[s] = nd;
       }
     } else {
-    // ---- occupied keys, compacted, then sorted ascending
+    // ---- occupied keys, compacted (keys into tile[], slots into voff[], both free until the
+    // kept blocks are numbered), then ranked: sslot[] lists the slots in ascending key order
     int32_t D = 0;
 #pragma unroll
     for (int t = 0; t < kHashCap / kWave; ++t) {
@@ -346,35 +352,32 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
       const uint64_t m = __ballot(k >= 0);
       if (k >= 0) {
         const int pos = D + __popcll(m & below_lane);
-        h.skey[pos] = k;
-        h.sslot[pos] = i;
+        h.tile[pos] = k;
+        h.voff[pos] = i;
       }
       D += __popcll(m);
     }
     // rank of each key = number of smaller keys (keys are unique); broadcast reads, no
-    // stage barriers. Ranked entries go to tile[] / voff[] first, then back to skey/sslot.
+    // stage barriers
     wsync();
     {
       int32_t mykey[kHashCap / kWave], myslot[kHashCap / kWave], rank[kHashCap / kWave];
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
         const int j = t * kWave + lane;
-        mykey[t] = j < D ? h.skey[j] : INT32_MAX;
-        myslot[t] = j < D ? h.sslot[j] : -1;
+        mykey[t] = j < D ? h.tile[j] : INT32_MAX;
+        myslot[t] = j < D ? h.voff[j] : -1;
         rank[t] = 0;
       }
       for (int i = 0; i < D; ++i) {
-        const int32_t ki = h.skey[i];
+        const int32_t ki = h.tile[i];
 #pragma unroll
         for (int t = 0; t < kHashCap / kWave; ++t) rank[t] += ki < mykey[t] ? 1 : 0;
       }
       wsync();
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
-        if (t * kWave + lane < D) {
-          h.skey[rank[t]] = mykey[t];
-          h.sslot[rank[t]] = myslot[t];
-        }
+        if (t * kWave + lane < D) h.sslot[rank[t]] = myslot[t];
       }
     }
     wsync();
@@ -408,7 +411,7 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
         h.tile[slot] = t;
         h.voff[slot] = val_base + vex;
         if (FILL) {
-          const int32_t key = h.skey[j];
+          const int32_t key = h.key[slot];
           if (kept) {
             o.codes[t] = code;
             o.coords[2 * (int64_t)t] = (int32_t)r0;
@@ -417,7 +420,7 @@ __global__ void __launch_bounds__(256) k_build_hash(BuildArgs a, FillOut o) {
           }
           if (o.blk_bc != nullptr) {
             o.blk_bc[d_base + j] = key;
-            o.blk_rmin[d_base + j] = (int8_t)h.rmin[slot];
+            o.blk_rmin[d_base + j] = (int8_t)(__builtin_ctzll(code) / bc);
             o.blk_tile[d_base + j] = kept ? t : -1;
           }
         }
@@ -604,8 +607,19 @@ static int group_lanes(int br) {
 template <bool FILL, typename V>
 static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
   if (o.big != nullptr) {
-    // waves: what a CU holds of each pass (LDS: 4 workgroups for fill, 7 for count)
-    const int64_t cap = (int64_t)device_info().num_cus * (FILL ? 16 : 28);
+    // waves: the fill pass fills the CUs once (fewer or more workgroups were slower), the
+    // count pass uses kHashCountWgPerCu
+    static std::atomic<int> per_cu{-1};
+    if (per_cu.load() < 0) {
+      int n = kHashCountWgPerCu;
+      if (FILL &&
+          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_build_hash<FILL, V>, 256, 0) !=
+               hipSuccess ||
+           n < 1))
+        n = 4;
+      per_cu.store(n);
+    }
+    const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
     const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
     hipLaunchKernelGGL((k_build_hash<FILL, V>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
                        st, a, o);
