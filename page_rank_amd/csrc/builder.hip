@@ -199,9 +199,9 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
 // distinct blocks are flagged and left to the merge kernel.
 // ---------------------------------------------------------------------------------------
 constexpr int kHashCap = 256;
-// Register budget of the fill pass: 6 waves per SIMD (80 VGPRs), which is also what its LDS
-// allows (6 workgroups of 4 waves per CU).
-constexpr int kHashWavesPerSimd = 6;
+// Register budget of the fill pass: 7 waves per SIMD (72 VGPRs), which is also what its LDS
+// allows (7 workgroups of 4 waves per CU).
+constexpr int kHashWavesPerSimd = 7;
 // Grid of the count pass, in workgroups per CU: 7 of the 8 its LDS allows (0.71 ms for the
 // whole 1M x 1M build, against 0.77 ms with 8 and with 4).
 constexpr int kHashCountWgPerCu = 7;
@@ -214,8 +214,8 @@ struct HashLds {
   static constexpr int kF = FILL ? kHashCap : 1;
   int32_t key[kHashCap];
   uint32_t lo[kHashCap], hi[kHashCap];
-  int32_t tile[kHashCap];  // per slot: tile index (fill) or 1 (count) when kept, -1 if not
-  int32_t voff[kF];        // per slot: offset of the tile's first value
+  int32_t voff[kHashCap];  // per slot when kept: offset of its first value (fill) or 1 (count);
+                           // -1 if not
   int32_t sslot[kF];       // slots in ascending key order
   int32_t rows[65];        // CSR start of each row of the stripe (block_rows + 1 values)
   int32_t rcnt[64];        // rejected entries per row so far
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         const bool occ = h.key[i] >= 0;
         const int32_t cnt = __popc(h.lo[i]) + __popc(h.hi[i]);
         const bool kept = occ && cnt >= a.min_nnz;
-        h.tile[i] = kept ? 1 : -1;
+        h.voff[i] = kept ? 1 : -1;
         nd += occ ? 1 : 0;
         nt += kept ? 1 : 0;
         nv += kept ? cnt : 0;
@@ -342,8 +342,8 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         o.nd[s] = nd;
       }
     } else {
-    // ---- occupied keys, compacted (keys into tile[], slots into voff[], both free until the
-    // kept blocks are numbered), then ranked: sslot[] lists the slots in ascending key order
+    // ---- occupied keys, compacted (keys into voff[], free until the kept blocks are numbered,
+    // slots into sslot[]), then ranked: sslot[] lists the slots in ascending key order
     int32_t D = 0;
 #pragma unroll
     for (int t = 0; t < kHashCap / kWave; ++t) {
@@ -352,8 +352,8 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       const uint64_t m = __ballot(k >= 0);
       if (k >= 0) {
         const int pos = D + __popcll(m & below_lane);
-        h.tile[pos] = k;
-        h.voff[pos] = i;
+        h.voff[pos] = k;
+        h.sslot[pos] = i;
       }
       D += __popcll(m);
     }
@@ -365,12 +365,12 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
         const int j = t * kWave + lane;
-        mykey[t] = j < D ? h.tile[j] : INT32_MAX;
-        myslot[t] = j < D ? h.voff[j] : -1;
+        mykey[t] = j < D ? h.voff[j] : INT32_MAX;
+        myslot[t] = j < D ? h.sslot[j] : -1;
         rank[t] = 0;
       }
       for (int i = 0; i < D; ++i) {
-        const int32_t ki = h.tile[i];
+        const int32_t ki = h.voff[i];
 #pragma unroll
         for (int t = 0; t < kHashCap / kWave; ++t) rank[t] += ki < mykey[t] ? 1 : 0;
       }
@@ -408,8 +408,7 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       const int32_t vex = vcar + vin - (kept ? cnt : 0);
       if (valid) {
         const int32_t t = kept ? (FILL ? tile_base + krank : 1) : -1;
-        h.tile[slot] = t;
-        h.voff[slot] = val_base + vex;
+        h.voff[slot] = kept ? val_base + vex : -1;
         if (FILL) {
           const int32_t key = h.key[slot];
           if (kept) {
@@ -445,7 +444,7 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         slot = hash_slot(b);
         while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
       }
-      const bool rej = valid && h.tile[slot] < 0;
+      const bool rej = valid && h.voff[slot] < 0;
       const uint64_t rm = __ballot(rej);
       // lanes of this row in this chunk: [ls, ...]
       const int ls = valid ? max(0, h.rows[r] - c) : 0;
