@@ -235,7 +235,8 @@ int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
                    uint64_t* near_count, void* stream);
 /* x0[pairs[m]] = val[m]. */
 int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, void* stream);
-/* Scale every column of x [N][K] to sum 1 (sums: K floats of scratch). */
+/* Scale every column of x [N][K] (K <= 1024) to sum 1. sums: K * 1025 floats of scratch; the
+ * first K receive the column sums, taken in a fixed order (repeated calls agree bit for bit). */
 int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream);
 
 /* ------------------------------------------------------------------------------------

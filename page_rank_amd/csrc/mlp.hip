@@ -363,14 +363,16 @@ __global__ void k_scatter_pairs(const int64_t* __restrict__ pairs, const float* 
   if (m < M) x0[pairs[m]] = val[m];
 }
 
-// Column sums of x [N][K]: a wave walks rows, lane k keeps column k (+64j) in registers; one
-// atomic add per column and workgroup at the end.
+// Column sums of x [N][K] in a fixed order, so that repeated runs agree bit for bit. A wave
+// walks rows (wave w of the grid takes rows w, w + nw, ...), lane k keeps column k (+64j) in
+// a register; the 4 waves of a workgroup are added in wave order into part[block][K], and
+// k_colsum_final adds the workgroups' partials in a fixed order.
+constexpr int kColsumBlocks = 1024;
 __global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ x, int64_t N, int64_t K,
-                                                float* __restrict__ sums) {
-  __shared__ float acc[1024];
-  for (int k = threadIdx.x; k < K; k += blockDim.x) acc[k] = 0.f;
-  __syncthreads();
+                                                float* __restrict__ part) {
+  __shared__ float acc[4][1024];
   const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
   const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t k0 = 0; k0 < K; k0 += 64) {
@@ -378,10 +380,40 @@ __global__ void __launch_bounds__(256) k_colsum(const float* __restrict__ x, int
     float a = 0.f;
     if (k < K)
       for (int64_t i = w0; i < N; i += nw) a += x[i * K + k];
-    if (k < K) atomicAdd(&acc[k], a);
+    if (k < K) acc[wid][k] = a;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < K; k += blockDim.x) atomicAdd(&sums[k], acc[k]);
+  for (int k = threadIdx.x; k < K; k += blockDim.x)
+    part[blockIdx.x * K + k] = ((acc[0][k] + acc[1][k]) + acc[2][k]) + acc[3][k];
+}
+// sums[k] = the nb partials of column k: wave w of the 16 takes partials w, w + 16, ... into
+// 8 accumulators (partial b into accumulator (b / 16) % 8), then the waves are added in order.
+__global__ void __launch_bounds__(1024) k_colsum_final(const float* __restrict__ part, int nb,
+                                                       int64_t K, float* __restrict__ sums) {
+  __shared__ float red[16][64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t k = blockIdx.x * 64LL + lane;
+  float a[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) a[j] = 0.f;
+  if (k < K) {
+    int b = w;
+    for (; b + 16 * 7 < nb; b += 16 * 8) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] += part[(int64_t)(b + 16 * j) * K + k];
+    }
+#pragma unroll
+    for (int j = 0; j < 7; ++j)
+      if (b + 16 * j < nb) a[j] += part[(int64_t)(b + 16 * j) * K + k];
+  }
+  red[w][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  __syncthreads();
+  if (w == 0 && k < K) {
+    float t = 0.f;
+    for (int i = 0; i < 16; ++i) t += red[i][lane];
+    sums[k] = t;
+  }
 }
 __global__ void k_colscale(float* __restrict__ x, int64_t N, int64_t K, const float* __restrict__ sums) {
   const int lane = threadIdx.x & 63;
@@ -542,11 +574,16 @@ int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, 
 }
 
 int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream) {
-  GL_REQUIRE(x && sums && K <= 1024, "bad arguments");
+  GL_REQUIRE(x && sums && K >= 1 && K <= 1024, "bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  GL_HIP(hipMemsetAsync(sums, 0, K * 4, st));
-  hipLaunchKernelGGL(k_colsum, dim3(nblocks(N * 64, 256, 1024)), dim3(256), 0, st, x, N, K, sums);
+  // sums[0, K): the column sums; sums[K, K + nb * K): the workgroups' partials
+  const int nb = nblocks(N * 64, 256, kColsumBlocks);
+  float* part = sums + K;
+  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, st, x, N, K, part);
   GL_LAUNCH_CHECK("k_colsum");
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((K + 63) / 64)), dim3(1024), 0, st, part,
+                     nb, K, sums);
+  GL_LAUNCH_CHECK("k_colsum_final");
   hipLaunchKernelGGL(k_colscale, dim3(nblocks(N * 64, 256, 16384)), dim3(256), 0, st, x, N, K,
                      sums);
   GL_LAUNCH_CHECK("k_colscale");

@@ -170,7 +170,8 @@ def predict(params, wi: WsInputs, near_cap=None):
         vals = forward(params, pair_features(wi, pairs))
         _native.check(L.gl_ws_scatter(pairs.data_ptr(), vals.data_ptr(), n_near, x0.data_ptr(), st),
                       "gl_ws_scatter")
-    sums = torch.empty(wi.K, dtype=torch.float32, device=dev)
+    # the column sums, then the workgroups' partials (at most 1024 of K floats)
+    sums = torch.empty(wi.K * 1025, dtype=torch.float32, device=dev)
     _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
                   "gl_ws_normalize")
     return x0, n_near
