@@ -2,10 +2,13 @@
  * graph_link_hip.h — C ABI of the MI355X (gfx950) Personalized-PageRank hot path.
  *
  * Every entry point:
- *   - takes plain device pointers, int64 sizes and an explicit hipStream_t (passed as void*);
+ *   - takes plain device pointers, int64 sizes and an explicit hipStream_t (passed as void*)
+ *     (the host SpMM, gl_pbr_spmm_host_*, takes host pointers and no stream);
  *   - returns GL_OK (0) or a GL_ERR_* code, with a message available from gl_last_error();
  *   - never throws across the ABI and never allocates device memory: scratch space is a
- *     caller-provided workspace whose size comes from the matching *_workspace_bytes query.
+ *     caller-provided workspace whose size comes from the matching *_workspace_bytes query;
+ *   - may be called from several host threads at once: each call that needs a stream of
+ *     its own (the PPR run) leases one that no other call uses until it returns.
  *
  * Matrix layout on the device is the reference's PBR ("packed block representation"):
  *   codes    u64[T]     occupancy bitmask of each BRxBC tile, bit (r*BC + c)
@@ -55,8 +58,10 @@ int gl_pbr_stripe_index(int64_t rows, int32_t block_rows, int64_t num_tiles,
  * bound at :142-145) and the launchers launch_pbr_spmm / launch_csr_spmm
  * (Backend/graph_link/kernels/pbr_kernel.cu:336-396, :460-478).
  *   mode GL_SPMM_FUSED : one stripe-owner kernel does tiles + remainder, plain stores.
- *   mode GL_SPMM_SPLIT : block kernel (b) on `stream`, CSR-remainder kernel (c) on the
- *                        library's side stream, joined back to `stream` with events.
+ *   mode GL_SPMM_SPLIT : block kernel (b), then the CSR-remainder kernel (c) adding into
+ *                        the same rows, both on `stream` (each owns whole rows of Y, so
+ *                        there are no atomics; the fused mode runs the two parts side by
+ *                        side inside one launch).
  * num_vals = offsets[num_tiles] (tile values), num_rem = rem_indptr[rows] (remainder nnz).
  * kw: RHS columns per launch (1,2,4,8 bit-lane kernel; 16,32,64 feature-lane kernel);
  *     0 lets the library choose. Block sizes 2x2, 4x4, 8x8 (as the reference).
@@ -105,7 +110,11 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* 
  * retire_mode: GL_RETIRE_TOGETHER stops every column when max_k residual_k < tol (the
  *   reference); GL_RETIRE_PER_COLUMN freezes each column at its own residual_k < tol.
  * residual_k = sum_i |X_{t+1}[i,k] - X_t[i,k]| (L1), as the reference.
- * X0: warm start [rows][K] (NULL = seed one-hot start, as the reference).
+ * X0: warm start [rows][K] (NULL = seed one-hot start, as the reference; X0 == X_out is a
+ *   warm start in place).
+ * K (seed sets) may be any positive count; the workspace grows with it.
+ * max_iter 0 runs no pass: X_out is the start vector, every column has 0 iterations and is
+ *   not converged (the reference's loop with max_iterations=0 returns (X, 0, False)).
  * Outputs: X_out [rows][K] (ld = K); iterations[K], converged[K] (device int32);
  *   history [max_iter][K] residuals (device, may be NULL); *total_iterations (host).
  * graph_iters > 0 captures that many iterations into one hipGraph and polls the device
@@ -270,6 +279,16 @@ int gl_pbr_to_csr_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_
                           const int32_t* rem_col_ind, const float* rem_vals, int32_t* indptr,
                           int32_t* indices, float* vals, int64_t nnz, void* workspace,
                           int64_t workspace_bytes, void* stream);
+/* The same for a float64 matrix (pbr_matrix.cpp:289-338 instantiated for double, bound at
+ * bindings.cpp:77-78); values are copied, never computed, so the output is exact. */
+int64_t gl_pbr_to_csr_f64_workspace_bytes(int64_t rows, int64_t nnz);
+int gl_pbr_to_csr_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                          int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                          const uint64_t* codes, const int32_t* coords, const int32_t* offsets,
+                          const double* data, const int32_t* rem_indptr,
+                          const int32_t* rem_col_ind, const double* rem_vals, int32_t* indptr,
+                          int32_t* indices, double* vals, int64_t nnz, void* workspace,
+                          int64_t workspace_bytes, void* stream);
 
 /* Top-k of every column of a dense row-major x [N][ld] (K columns): values [k][K] and
  * rows [k][K] (int64), largest first, equal values by ascending row (deterministic).
@@ -293,20 +312,91 @@ int gl_pbr_spmm_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
                         const double* rem_vals, const double* X, int64_t ldx, int64_t K,
                         double* Y, int64_t ldy, int32_t accumulate, void* stream);
 
+/* ------------------------------------------------------------------------------------
+ * The reference's per-iteration PPR kernels as separate calls (FP32 and FP64), for a host
+ * loop of SpMM + update per iteration (Backend/graph_link/graph_link/__init__.py:240-260).
+ * X, Y are device [N][K] row-major (ld = K). Sums over rows (residuals, column sums) are
+ * per-block partials in the workspace, added in block order (the reference adds float
+ * atomics), so repeated calls agree bit for bit.
+ * ---------------------------------------------------------------------------------- */
 /* X [N][K] = one-hot columns, X[i][k] = (i == source_nodes[k]); the whole of X is written.
- * Replaces init_ppr_cuda_double (Backend/graph_link/kernels/ppr_kernel.cu:199-216, bound at
- * bindings_kernels.cu:149), which also needs X zeroed beforehand. */
+ * Replaces init_ppr_cuda_float / init_ppr_cuda_double
+ * (Backend/graph_link/kernels/ppr_kernel.cu:199-216, bound at bindings_kernels.cu:148-149),
+ * which also need X zeroed beforehand. */
+int gl_ppr_init_f32(float* X, const int32_t* source_nodes, int64_t N, int64_t K, void* stream);
 int gl_ppr_init_f64(double* X, const int32_t* source_nodes, int64_t N, int64_t K,
                     void* stream);
 
-/* X <- alpha Y + (1 - alpha) e_src in place; errors[k] = L1 norm of column k's update.
- * Replaces ppr_update_normalized_cuda_double (ppr_kernel.cu:130-187, bound at
- * bindings_kernels.cu:161). The residual is summed in a fixed order (the reference uses
- * atomics). The workspace holds per-block partial sums. */
+/* Workspace of the update and column-sum calls below (per-block partial sums). */
+int64_t gl_ppr_update_f32_workspace_bytes(int64_t N, int64_t K);
 int64_t gl_ppr_update_f64_workspace_bytes(int64_t N, int64_t K);
+
+/* X <- alpha Y + (1 - alpha) e_src in place; errors[k] = L1 norm of column k's update.
+ * Replaces ppr_update_normalized_cuda_float / _double (ppr_kernel.cu:130-187, bound at
+ * bindings_kernels.cu:160-161). */
+int gl_ppr_update_normalized_f32(const float* Y, float* X, const int32_t* source_nodes,
+                                 float alpha, int64_t N, int64_t K, float* errors,
+                                 void* workspace, int64_t workspace_bytes, void* stream);
 int gl_ppr_update_normalized_f64(const double* Y, double* X, const int32_t* source_nodes,
                                  double alpha, int64_t N, int64_t K, double* errors,
                                  void* workspace, int64_t workspace_bytes, void* stream);
+
+/* col_sums[k] = sum_i Y[i][k] (overwritten). Replaces missing_mass_cuda_float / _double
+ * (ppr_kernel.cu:13-48, bound at bindings_kernels.cu:152-153). */
+int gl_ppr_missing_mass_f32(const float* Y, float* col_sums, int64_t N, int64_t K,
+                            void* workspace, int64_t workspace_bytes, void* stream);
+int gl_ppr_missing_mass_f64(const double* Y, double* col_sums, int64_t N, int64_t K,
+                            void* workspace, int64_t workspace_bytes, void* stream);
+
+/* X_next = alpha Y + [i == src_k] (alpha (1 - col_sums[k]) + (1 - alpha)): the step for a
+ * matrix whose dangling columns are empty, their mass returned to the seed (networkx's
+ * default). errors[k] += L1 norm of X_next - X_cur (added, as the reference, which leaves
+ * zeroing errors to the caller). Replaces ppr_update_cuda_float / _double
+ * (ppr_kernel.cu:50-117, bound at bindings_kernels.cu:156-157). */
+int gl_ppr_update_f32(const float* Y, const float* X_cur, float* X_next,
+                      const int32_t* source_nodes, const float* col_sums, float alpha,
+                      int64_t N, int64_t K, float* errors, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+int gl_ppr_update_f64(const double* Y, const double* X_cur, double* X_next,
+                      const int32_t* source_nodes, const double* col_sums, double alpha,
+                      int64_t N, int64_t K, double* errors, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------------------
+ * Host (CPU) SpMM: Y = A X with the PBR arrays, X and Y in HOST memory (row-major, leading
+ * dimensions ldx / ldy), for the four index / value types of the reference's classes.
+ * Replaces pbr_batched_matmul_cpu (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:17-68,
+ * bound at Backend/graph_link/bindings/bindings.cpp:65-68), which pbr_matmul uses for CPU
+ * tensors (Backend/graph_link/graph_link/__init__.py:130-134). Any block size with
+ * block_rows * block_cols <= 64 and any tile order. Every row adds its tiles' products in
+ * tile and bit order, then its remainder entries, each product rounded on its own, so Y is
+ * bit-identical to the reference's. `threads` host threads (0 = all cores) own disjoint
+ * stripe ranges. Needs no GPU.
+ * ---------------------------------------------------------------------------------- */
+int gl_pbr_spmm_host_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                             int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                             const int32_t* offsets, const float* data,
+                             const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                             const float* rem_vals, const float* X, int64_t ldx, int64_t K,
+                             float* Y, int64_t ldy, int32_t threads);
+int gl_pbr_spmm_host_i64_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                             int64_t num_tiles, const uint64_t* codes, const int64_t* coords,
+                             const int64_t* offsets, const float* data,
+                             const int64_t* rem_indptr, const int64_t* rem_col_ind,
+                             const float* rem_vals, const float* X, int64_t ldx, int64_t K,
+                             float* Y, int64_t ldy, int32_t threads);
+int gl_pbr_spmm_host_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                             int64_t num_tiles, const uint64_t* codes, const int32_t* coords,
+                             const int32_t* offsets, const double* data,
+                             const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                             const double* rem_vals, const double* X, int64_t ldx, int64_t K,
+                             double* Y, int64_t ldy, int32_t threads);
+int gl_pbr_spmm_host_i64_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                             int64_t num_tiles, const uint64_t* codes, const int64_t* coords,
+                             const int64_t* offsets, const double* data,
+                             const int64_t* rem_indptr, const int64_t* rem_col_ind,
+                             const double* rem_vals, const double* X, int64_t ldx, int64_t K,
+                             double* Y, int64_t ldy, int32_t threads);
 
 #ifdef __cplusplus
 }

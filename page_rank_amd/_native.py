@@ -101,19 +101,35 @@ _PROTOS = {
     "gl_csr_normalize_columns_f32": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
                                                     c_i64, c_p]),
     "gl_pbr_to_csr_workspace_bytes": (c_i64, [c_i64, c_i64]),
-    "gl_pbr_to_csr_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64, c_i64,
-                                             c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
-                                             c_i64, c_p, c_i64, c_p]),
+    "gl_pbr_to_csr_f64_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    **{f"gl_pbr_to_csr_i32_{ft}": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64,
+                                                  c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                  c_p, c_p, c_p, c_i64, c_p, c_i64, c_p])
+       for ft in ("f32", "f64")},
     "gl_topk_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "gl_topk_columns_f32": (ctypes.c_int, [c_p, c_i64, c_i64, c_i64, c_i64, c_p, c_p, c_p, c_i64,
                                            c_p]),
     "gl_pbr_spmm_i32_f64": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_p, c_p, c_p,
                                            c_p, c_p, c_p, c_p, c_p, c_p, c_i64, c_i64, c_p,
                                            c_i64, c_i32, c_p]),
+    "gl_ppr_init_f32": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p]),
     "gl_ppr_init_f64": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p]),
+    "gl_ppr_update_f32_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "gl_ppr_update_f64_workspace_bytes": (c_i64, [c_i64, c_i64]),
+    "gl_ppr_update_normalized_f32": (ctypes.c_int, [c_p, c_p, c_p, c_f32, c_i64,
+                                                    c_i64, c_p, c_p, c_i64, c_p]),
     "gl_ppr_update_normalized_f64": (ctypes.c_int, [c_p, c_p, c_p, ctypes.c_double, c_i64,
                                                     c_i64, c_p, c_p, c_i64, c_p]),
+    "gl_ppr_missing_mass_f32": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p, c_i64, c_p]),
+    "gl_ppr_missing_mass_f64": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p, c_i64, c_p]),
+    "gl_ppr_update_f32": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_p, c_f32, c_i64, c_i64, c_p,
+                                         c_p, c_i64, c_p]),
+    "gl_ppr_update_f64": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_p, ctypes.c_double, c_i64,
+                                         c_i64, c_p, c_p, c_i64, c_p]),
+    **{f"gl_pbr_spmm_host_{it}_{ft}": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64,
+                                                      c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                      c_p, c_i64, c_i64, c_p, c_i64, c_i32])
+       for it in ("i32", "i64") for ft in ("f32", "f64")},
 }
 
 _lib = None

@@ -124,8 +124,10 @@ def pbr_to_csr(pbr_mat, indptr, indices, data):
     if meta is None:
         tmp = pbr_mat.to(_device(None))
         meta = get_pbr_gpu_meta(tmp)
-    if meta["data"].dtype != torch.float32:
-        raise TypeError("pbr_to_csr on the GPU supports float32 matrices")
+    vdt = meta["data"].dtype
+    if vdt not in (torch.float32, torch.float64):
+        raise TypeError(f"Unsupported value dtype {vdt}")
+    f64 = vdt == torch.float64
     dev = meta["codes"].device
     n = int(np.asarray(indices).shape[0]) if not isinstance(indices, torch.Tensor) \
         else indices.numel()
@@ -134,21 +136,23 @@ def pbr_to_csr(pbr_mat, indptr, indices, data):
     rows = pbr_mat.rows
     ip = torch.empty(rows + 1, dtype=torch.int32, device=dev)
     ix = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
-    vx = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+    vx = torch.empty(max(n, 1), dtype=vdt, device=dev)
     L = _native.lib()
-    wsb = L.gl_pbr_to_csr_workspace_bytes(rows, n)
+    wsb = (L.gl_pbr_to_csr_f64_workspace_bytes if f64 else L.gl_pbr_to_csr_workspace_bytes)(
+        rows, n)
     if wsb < 0:
         raise ValueError("invalid sizes for pbr_to_csr")
     ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+    fn = L.gl_pbr_to_csr_i32_f64 if f64 else L.gl_pbr_to_csr_i32_f32
     with torch.cuda.device(dev):
-        _native.check(L.gl_pbr_to_csr_i32_f32(
+        _native.check(fn(
             rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
             meta["codes"].numel(), meta["data"].numel(), meta["rem_col_ind"].numel(),
             meta["codes"].data_ptr(), meta["coords"].data_ptr(), meta["offsets"].data_ptr(),
             meta["data"].data_ptr(), meta["rem_indptr"].data_ptr(),
             meta["rem_col_ind"].data_ptr(), meta["rem_vals"].data_ptr(), ip.data_ptr(),
             ix.data_ptr(), vx.data_ptr(), n, ws.data_ptr(), wsb,
-            _native.stream_handle(dev)), "gl_pbr_to_csr_i32_f32")
+            _native.stream_handle(dev)), "gl_pbr_to_csr")
     for src, dst in ((ip, indptr), (ix[:n], indices), (vx[:n], data)):
         if isinstance(dst, torch.Tensor):
             dst.copy_(src.to(dst.dtype))
@@ -159,11 +163,11 @@ def pbr_to_csr(pbr_mat, indptr, indices, data):
 
 
 def pbr_to_scipy(pbr_mat) -> sp.csr_matrix:
-    """The PBR matrix as a canonical scipy CSR (float32)."""
+    """The PBR matrix as a canonical scipy CSR of its value type."""
     n = pbr_mat.total_nnz
     indptr = np.empty(pbr_mat.rows + 1, np.int32)
     indices = np.empty(n, np.int32)
-    data = np.empty(n, np.float32)
+    data = np.empty(n, np.dtype(pbr_mat.scalar_dtype))
     pbr_to_csr(pbr_mat, indptr, indices, data)
     return sp.csr_matrix((data, indices, indptr), shape=(pbr_mat.rows, pbr_mat.cols))
 

@@ -1,6 +1,7 @@
 // C ABI: error reporting, device queries, the stripe index and the SpMM entry points.
 #include <cstdarg>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,17 +32,12 @@ namespace {
 struct PerDevice {
   bool init = false;
   DeviceInfo info;
-  hipStream_t side = nullptr;
-  hipStream_t cap = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<StreamSet*> free_sets;  // stream sets not leased by any call
 };
 std::mutex g_mu;
-std::vector<PerDevice> g_dev;
+std::deque<PerDevice> g_dev;  // a deque: growing it keeps references to its elements
 
-PerDevice& per_device() {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_mu);
+PerDevice& per_device_locked(int dev) {
   if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
   PerDevice& p = g_dev[dev];
   if (!p.init) {
@@ -52,19 +48,54 @@ PerDevice& per_device() {
       p.info.clock_khz = prop.clockRate;
       p.info.hbm_bytes = (int64_t)prop.totalGlobalMem;
     }
-    (void)hipStreamCreateWithFlags(&p.side, hipStreamNonBlocking);
-    (void)hipStreamCreateWithFlags(&p.cap, hipStreamNonBlocking);
-    for (auto& e : p.ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     p.init = true;
   }
   return p;
 }
 }  // namespace
 
-const DeviceInfo& device_info() { return per_device().info; }
-hipStream_t side_stream() { return per_device().side; }
-hipStream_t capture_stream() { return per_device().cap; }
-hipEvent_t lib_event(int i) { return per_device().ev[i & 3]; }
+const DeviceInfo& device_info() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  return per_device_locked(dev).info;
+}
+
+// Stream sets are leased for the length of one call and returned to a per-device free
+// list, so two calls that run at once (threads of a server) never share a stream or an
+// event. A returned set may still hold queued work of the call that used it; the next
+// call's work then queues behind it, and an event wait issued earlier is not affected by a
+// later record of the same event, so reuse is safe. Sets live until the process ends.
+StreamSet* acquire_stream_set() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    PerDevice& p = per_device_locked(dev);
+    if (!p.free_sets.empty()) {
+      StreamSet* s = p.free_sets.back();
+      p.free_sets.pop_back();
+      return s;
+    }
+  }
+  StreamSet* s = new StreamSet;
+  s->device = dev;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    set_error("HIP error %d (%s) creating a stream set", (int)e, hipGetErrorString(e));
+    delete s;  // its stream or events, if created, stay (this path only runs on failure)
+    return nullptr;
+  }
+  return s;
+}
+
+void release_stream_set(StreamSet* s) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  per_device_locked(s->device).free_sets.push_back(s);
+}
 
 // SpMM grids: up to 32 workgroups per CU, more than the device holds at once, so the
 // hardware hands out the later ones as earlier ones finish and every CU stays busy to the
@@ -121,7 +152,7 @@ extern "C" {
 
 const char* gl_last_error(void) { return t_last_error.c_str(); }
 
-int32_t gl_abi_version(void) { return 4; }
+int32_t gl_abi_version(void) { return 5; }
 
 int gl_device_info(int32_t* num_cus, int32_t* lds_bytes_per_cu, int32_t* clock_khz,
                    int64_t* hbm_bytes) {
@@ -246,16 +277,12 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
                                    codes, coords, offsets, data, stripe_ptr, X, ldx, K, Y, ldy,
                                    accumulate, kw, stream);
     if (rc) return rc;
-    // the remainder kernel runs on the library's side stream, ordered after the block
-    // kernel (both own whole rows of Y, so the second one adds without atomics)
-    hipStream_t side = side_stream();
-    GL_HIP(hipEventRecord(lib_event(0), st));
-    GL_HIP(hipStreamWaitEvent(side, lib_event(0), 0));
+    // the remainder kernel follows the block kernel on the same stream: both own whole
+    // rows of Y, so the second one adds without atomics (the fused mode is the one that
+    // runs tiles and remainder side by side, inside one launch)
     rc = gl_csr_spmm_i32_f32(rows, cols, num_rem, rem_indptr, rem_col_ind, rem_vals, X, ldx, K,
-                             Y, ldy, 1, kw, side);
+                             Y, ldy, 1, kw, stream);
     if (rc) return rc;
-    GL_HIP(hipEventRecord(lib_event(1), side));
-    GL_HIP(hipStreamWaitEvent(st, lib_event(1), 0));
     return GL_OK;
   }
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, num_rem,

@@ -202,12 +202,14 @@ __global__ void k_p2c_count(int64_t rows, int64_t cols, int32_t br, int32_t bc, 
   }
 }
 
+// Values are payloads (copied, never computed): V is float or double.
+template <typename V>
 __global__ void k_p2c_fill(int64_t rows, int32_t br, int32_t bc, int64_t T,
                            const uint64_t* __restrict__ codes, const int32_t* __restrict__ coords,
-                           const int32_t* __restrict__ offsets, const float* __restrict__ data,
+                           const int32_t* __restrict__ offsets, const V* __restrict__ data,
                            const int32_t* __restrict__ rem_ptr, const int32_t* __restrict__ rem_col,
-                           const float* __restrict__ rem_val, int32_t* __restrict__ cursor,
-                           int32_t* __restrict__ col_out, float* __restrict__ val_out) {
+                           const V* __restrict__ rem_val, int32_t* __restrict__ cursor,
+                           int32_t* __restrict__ col_out, V* __restrict__ val_out) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < T;
        t += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t c = codes[t];
@@ -241,22 +243,24 @@ __global__ void k_p2c_fill(int64_t rows, int32_t br, int32_t bc, int64_t T,
   }
 }
 
+template <typename V>
 struct P2cWs {
   int32_t* cnt;     // rows + 1 counts, scanned in place into indptr order
   int32_t* cursor;  // rows
   int32_t* flag;
   int32_t* keys;    // nnz (unsorted columns)
-  float* vals;      // nnz (unsorted values)
+  V* vals;          // nnz (unsorted values)
   void* tmp;
   size_t tmp_bytes;
   int64_t bytes;
 };
 
-static int p2c_carve(char* base, int64_t rows, int64_t nnz, P2cWs& w) {
+template <typename V>
+static int p2c_carve(char* base, int64_t rows, int64_t nnz, P2cWs<V>& w) {
   size_t sort_b = 0, scan_b = 0;
   GL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
-      nullptr, sort_b, (const int32_t*)nullptr, (int32_t*)nullptr, (const float*)nullptr,
-      (float*)nullptr, (int)std::max<int64_t>(nnz, 1), (int)std::max<int64_t>(rows, 1),
+      nullptr, sort_b, (const int32_t*)nullptr, (int32_t*)nullptr, (const V*)nullptr,
+      (V*)nullptr, (int)std::max<int64_t>(nnz, 1), (int)std::max<int64_t>(rows, 1),
       (const int32_t*)nullptr, (const int32_t*)nullptr, 0, 31));
   GL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_b, (int32_t*)nullptr,
                                           (int32_t*)nullptr, (int)(rows + 1)));
@@ -270,10 +274,70 @@ static int p2c_carve(char* base, int64_t rows, int64_t nnz, P2cWs& w) {
   w.cursor = (int32_t*)take(rows * 4);
   w.flag = (int32_t*)take(4);
   w.keys = (int32_t*)take(nnz * 4);
-  w.vals = (float*)take(nnz * 4);
+  w.vals = (V*)take(nnz * (int64_t)sizeof(V));
   w.tmp_bytes = std::max(sort_b, scan_b);
   w.tmp = take((int64_t)w.tmp_bytes);
   w.bytes = off;
+  return GL_OK;
+}
+
+template <typename V>
+int64_t p2c_workspace_bytes(int64_t rows, int64_t nnz) {
+  if (rows < 0 || nnz < 0 || rows >= INT32_MAX || nnz >= INT32_MAX) return -1;
+  P2cWs<V> w;
+  if (p2c_carve<V>(nullptr, rows, nnz, w) != GL_OK) return -1;
+  return w.bytes;
+}
+
+template <typename V>
+int pbr_to_csr(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+               int64_t num_tiles, int64_t num_vals, int64_t num_rem, const uint64_t* codes,
+               const int32_t* coords, const int32_t* offsets, const V* data,
+               const int32_t* rem_indptr, const int32_t* rem_col_ind, const V* rem_vals,
+               int32_t* indptr, int32_t* indices, V* vals, int64_t nnz, void* ws,
+               int64_t ws_bytes, void* stream) {
+  GL_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && rows < INT32_MAX && nnz < INT32_MAX,
+             "bad sizes");
+  GL_REQUIRE(block_rows >= 1 && block_cols >= 1 && block_rows * block_cols <= 64,
+             "block %dx%d does not fit a 64-bit code", block_rows, block_cols);
+  GL_REQUIRE(num_tiles >= 0 && num_vals >= 0 && num_rem >= 0, "bad sizes");
+  GL_REQUIRE(num_tiles == 0 || (codes && coords && offsets && data), "null tile arrays");
+  GL_REQUIRE(num_rem == 0 || (rem_col_ind && rem_vals), "null remainder arrays");
+  GL_REQUIRE(rem_indptr && indptr && (nnz == 0 || (indices && vals)), "null CSR arrays");
+  P2cWs<V> w;
+  int rc = p2c_carve<V>((char*)ws, rows, nnz, w);
+  if (rc) return rc;
+  GL_REQUIRE(ws != nullptr && ws_bytes >= w.bytes, "workspace too small: %lld < %lld bytes",
+             (long long)ws_bytes, (long long)w.bytes);
+  hipStream_t st = (hipStream_t)stream;
+  GL_HIP(hipMemsetAsync(w.cnt, 0, (rows + 1) * 4, st));
+  GL_HIP(hipMemsetAsync(w.flag, 0, 4, st));
+  hipLaunchKernelGGL(k_p2c_count, dim3(nblk(std::max(num_tiles, rows))), dim3(256), 0, st, rows,
+                     cols, block_rows, block_cols, num_tiles, num_vals, num_rem, codes, coords,
+                     offsets, rem_indptr, rem_col_ind, w.cnt, w.flag);
+  GL_LAUNCH_CHECK("k_p2c_count");
+  size_t tb = w.tmp_bytes;
+  GL_HIP(hipcub::DeviceScan::ExclusiveSum(w.tmp, tb, w.cnt, indptr, (int)(rows + 1), st));
+  int32_t h[2] = {0, 0};
+  GL_HIP(hipMemcpyAsync(&h[0], indptr + rows, 4, hipMemcpyDeviceToHost, st));
+  GL_HIP(hipMemcpyAsync(&h[1], w.flag, 4, hipMemcpyDeviceToHost, st));
+  GL_HIP(hipStreamSynchronize(st));
+  GL_REQUIRE(!(h[1] & 1), "a tile has a set bit outside the %lldx%lld matrix",
+             (long long)rows, (long long)cols);
+  GL_REQUIRE(!(h[1] & 2), "tile offsets point outside the %lld tile values", (long long)num_vals);
+  GL_REQUIRE(!(h[1] & 4), "the remainder CSR has entries outside the matrix");
+  GL_REQUIRE(h[0] == nnz, "the matrix holds %d entries but the output has room for %lld",
+             h[0], (long long)nnz);
+  if (nnz == 0) return GL_OK;
+  GL_HIP(hipMemcpyAsync(w.cursor, indptr, rows * 4, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_p2c_fill<V>, dim3(nblk(std::max(num_tiles, rows))), dim3(256), 0, st, rows,
+                     block_rows, block_cols, num_tiles, codes, coords, offsets, data, rem_indptr,
+                     rem_col_ind, rem_vals, w.cursor, w.keys, w.vals);
+  GL_LAUNCH_CHECK("k_p2c_fill");
+  tb = w.tmp_bytes;
+  GL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(w.tmp, tb, w.keys, indices, w.vals, vals,
+                                                     (int)nnz, (int)rows, indptr, indptr + 1, 0,
+                                                     key_bits(cols), st));
   return GL_OK;
 }
 
@@ -326,10 +390,11 @@ int gl_csr_normalize_columns_f32(int64_t cols, int64_t nnz, const int32_t* indic
 }
 
 int64_t gl_pbr_to_csr_workspace_bytes(int64_t rows, int64_t nnz) {
-  if (rows < 0 || nnz < 0 || rows >= INT32_MAX || nnz >= INT32_MAX) return -1;
-  P2cWs w;
-  if (p2c_carve(nullptr, rows, nnz, w) != GL_OK) return -1;
-  return w.bytes;
+  return p2c_workspace_bytes<float>(rows, nnz);
+}
+
+int64_t gl_pbr_to_csr_f64_workspace_bytes(int64_t rows, int64_t nnz) {
+  return p2c_workspace_bytes<double>(rows, nnz);
 }
 
 int gl_pbr_to_csr_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
@@ -339,49 +404,21 @@ int gl_pbr_to_csr_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_
                           const int32_t* rem_col_ind, const float* rem_vals, int32_t* indptr,
                           int32_t* indices, float* vals, int64_t nnz, void* ws, int64_t ws_bytes,
                           void* stream) {
-  GL_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0 && rows < INT32_MAX && nnz < INT32_MAX,
-             "bad sizes");
-  GL_REQUIRE(block_rows >= 1 && block_cols >= 1 && block_rows * block_cols <= 64,
-             "block %dx%d does not fit a 64-bit code", block_rows, block_cols);
-  GL_REQUIRE(num_tiles >= 0 && num_vals >= 0 && num_rem >= 0, "bad sizes");
-  GL_REQUIRE(num_tiles == 0 || (codes && coords && offsets && data), "null tile arrays");
-  GL_REQUIRE(num_rem == 0 || (rem_col_ind && rem_vals), "null remainder arrays");
-  GL_REQUIRE(rem_indptr && indptr && (nnz == 0 || (indices && vals)), "null CSR arrays");
-  P2cWs w;
-  int rc = p2c_carve((char*)ws, rows, nnz, w);
-  if (rc) return rc;
-  GL_REQUIRE(ws != nullptr && ws_bytes >= w.bytes, "workspace too small: %lld < %lld bytes",
-             (long long)ws_bytes, (long long)w.bytes);
-  hipStream_t st = (hipStream_t)stream;
-  GL_HIP(hipMemsetAsync(w.cnt, 0, (rows + 1) * 4, st));
-  GL_HIP(hipMemsetAsync(w.flag, 0, 4, st));
-  hipLaunchKernelGGL(k_p2c_count, dim3(nblk(std::max(num_tiles, rows))), dim3(256), 0, st, rows,
-                     cols, block_rows, block_cols, num_tiles, num_vals, num_rem, codes, coords,
-                     offsets, rem_indptr, rem_col_ind, w.cnt, w.flag);
-  GL_LAUNCH_CHECK("k_p2c_count");
-  size_t tb = w.tmp_bytes;
-  GL_HIP(hipcub::DeviceScan::ExclusiveSum(w.tmp, tb, w.cnt, indptr, (int)(rows + 1), st));
-  int32_t h[2] = {0, 0};
-  GL_HIP(hipMemcpyAsync(&h[0], indptr + rows, 4, hipMemcpyDeviceToHost, st));
-  GL_HIP(hipMemcpyAsync(&h[1], w.flag, 4, hipMemcpyDeviceToHost, st));
-  GL_HIP(hipStreamSynchronize(st));
-  GL_REQUIRE(!(h[1] & 1), "a tile has a set bit outside the %lldx%lld matrix",
-             (long long)rows, (long long)cols);
-  GL_REQUIRE(!(h[1] & 2), "tile offsets point outside the %lld tile values", (long long)num_vals);
-  GL_REQUIRE(!(h[1] & 4), "the remainder CSR has entries outside the matrix");
-  GL_REQUIRE(h[0] == nnz, "the matrix holds %d entries but the output has room for %lld",
-             h[0], (long long)nnz);
-  if (nnz == 0) return GL_OK;
-  GL_HIP(hipMemcpyAsync(w.cursor, indptr, rows * 4, hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_p2c_fill, dim3(nblk(std::max(num_tiles, rows))), dim3(256), 0, st, rows,
-                     block_rows, block_cols, num_tiles, codes, coords, offsets, data, rem_indptr,
-                     rem_col_ind, rem_vals, w.cursor, w.keys, w.vals);
-  GL_LAUNCH_CHECK("k_p2c_fill");
-  tb = w.tmp_bytes;
-  GL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(w.tmp, tb, w.keys, indices, w.vals, vals,
-                                                     (int)nnz, (int)rows, indptr, indptr + 1, 0,
-                                                     key_bits(cols), st));
-  return GL_OK;
+  return pbr_to_csr<float>(rows, cols, block_rows, block_cols, num_tiles, num_vals, num_rem,
+                           codes, coords, offsets, data, rem_indptr, rem_col_ind, rem_vals,
+                           indptr, indices, vals, nnz, ws, ws_bytes, stream);
+}
+
+int gl_pbr_to_csr_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
+                          int64_t num_tiles, int64_t num_vals, int64_t num_rem,
+                          const uint64_t* codes, const int32_t* coords,
+                          const int32_t* offsets, const double* data,
+                          const int32_t* rem_indptr, const int32_t* rem_col_ind,
+                          const double* rem_vals, int32_t* indptr, int32_t* indices,
+                          double* vals, int64_t nnz, void* ws, int64_t ws_bytes, void* stream) {
+  return pbr_to_csr<double>(rows, cols, block_rows, block_cols, num_tiles, num_vals, num_rem,
+                            codes, coords, offsets, data, rem_indptr, rem_col_ind, rem_vals,
+                            indptr, indices, vals, nnz, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

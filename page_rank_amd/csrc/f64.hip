@@ -1,7 +1,6 @@
-// FP64 variants of the path: the SpMM of a float64 PBR matrix and the reference's PPR
-// init / update step in double. They keep float64 matrices working end to end, as the
-// reference's templated kernels do (pbr_spmm_cuda_i32_f64, init_ppr_cuda_double,
-// ppr_update_normalized_cuda_double); the FP32 kernels remain the fast path.
+// FP64 SpMM of a float64 PBR matrix. It keeps float64 matrices working end to end, as the
+// reference's templated kernels do (pbr_spmm_cuda_i32_f64); the FP32 kernels remain the fast
+// path. The PPR init / update steps in both precisions are in ppr_step.hip.
 //
 // SpMM: one wave per stripe (XCD-aware persistent grid). Lane (r, q) owns row r of the
 // stripe and RHS columns q, q + L, ... (L = 64 / B lanes per row); it walks the stripe's
@@ -73,83 +72,6 @@ __global__ void __launch_bounds__(256) k_spmm_f64(PbrView64 m, const double* __r
   }
 }
 
-// X = one-hot columns: X[i][k] = (i == src[k]).
-__global__ void k_ppr_init_f64(double* __restrict__ X, const int32_t* __restrict__ src,
-                               int64_t N, int64_t K) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * K;
-       e += (int64_t)gridDim.x * blockDim.x)
-    X[e] = (e / K == (int64_t)src[e % K]) ? 1.0 : 0.0;
-}
-
-// X <- alpha Y + (1 - alpha) e_src in place, with per-block partial L1 norms of the update
-// for every column. Block b owns rows [b * rpb, (b + 1) * rpb). A wave covers W = min(K, 64)
-// columns and RP = 64 / W rows per step; the four waves of the block interleave rows. The
-// partial of a column is summed in a fixed order, so the residual does not depend on timing.
-__global__ void __launch_bounds__(256) k_ppr_update_f64(const double* __restrict__ Y,
-                                                        double* __restrict__ X,
-                                                        const int32_t* __restrict__ src,
-                                                        double alpha, int64_t N, int64_t K,
-                                                        int64_t rpb,
-                                                        double* __restrict__ partial) {
-  __shared__ double red[256];
-  const int W = (int)(K < kWave ? K : kWave);
-  const int RP = kWave / W;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
-  const int lc = lane % W, lr = lane / W;
-  const bool lane_ok = lr < RP;
-  const int64_t r0 = blockIdx.x * rpb;
-  const int64_t r1 = r0 + rpb < N ? r0 + rpb : N;
-  const double comp = 1.0 - alpha;
-  for (int64_t k0 = 0; k0 < K; k0 += W) {
-    const int64_t k = k0 + lc;
-    const bool ok = lane_ok && k < K;
-    const int64_t sk = ok ? (int64_t)src[k] : -1;
-    double e = 0.0;
-    if (ok) {
-      for (int64_t i = r0 + wid * RP + lr; i < r1; i += 4 * RP) {
-        const int64_t ix = i * K + k;
-        double nv = alpha * Y[ix];
-        if (i == sk) nv += comp;
-        e += fabs(nv - X[ix]);
-        X[ix] = nv;
-      }
-    }
-    red[threadIdx.x] = e;
-    __syncthreads();
-    if (threadIdx.x < W && k0 + threadIdx.x < K) {
-      double sum = 0.0;
-      for (int w = 0; w < 4; ++w)
-        for (int rr = 0; rr < RP; ++rr) sum += red[w * kWave + rr * W + threadIdx.x];
-      partial[blockIdx.x * K + k0 + threadIdx.x] = sum;
-    }
-    __syncthreads();
-  }
-}
-
-// errors[k] = sum over blocks, in block order.
-__global__ void k_ppr_err_f64(const double* __restrict__ partial, int64_t nb, int64_t K,
-                              double* __restrict__ errors) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < K;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int64_t b = 0; b < nb; ++b) s += partial[b * K + k];
-    errors[k] = s;
-  }
-}
-
-constexpr int64_t kUpdBlocks = 1024;
-
-static void update_shape(int64_t N, int64_t K, int64_t& nb, int64_t& rpb) {
-  const int64_t W = K < kWave ? K : kWave;
-  const int64_t step = 4 * (kWave / W);  // rows one block advances per loop step
-  rpb = (N + kUpdBlocks - 1) / kUpdBlocks;
-  rpb = ((rpb + step - 1) / step) * step;
-  if (rpb < step) rpb = step;
-  nb = (N + rpb - 1) / rpb;
-  if (nb < 1) nb = 1;
-}
-
 }  // namespace gl
 
 using namespace gl;
@@ -193,43 +115,6 @@ int gl_pbr_spmm_i32_f64(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
       break;
   }
   GL_LAUNCH_CHECK("k_spmm_f64");
-  return GL_OK;
-}
-
-int gl_ppr_init_f64(double* X, const int32_t* source_nodes, int64_t N, int64_t K,
-                    void* stream) {
-  GL_REQUIRE(N >= 1 && K >= 1, "bad sizes (N=%lld K=%lld)", (long long)N, (long long)K);
-  GL_REQUIRE(X && source_nodes, "null pointer");
-  const int64_t nb = std::min<int64_t>((N * K + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_ppr_init_f64, dim3((unsigned)nb), dim3(256), 0, (hipStream_t)stream, X,
-                     source_nodes, N, K);
-  GL_LAUNCH_CHECK("k_ppr_init_f64");
-  return GL_OK;
-}
-
-int64_t gl_ppr_update_f64_workspace_bytes(int64_t N, int64_t K) {
-  if (N < 1 || K < 1) return -1;
-  int64_t nb, rpb;
-  update_shape(N, K, nb, rpb);
-  return nb * K * 8;
-}
-
-int gl_ppr_update_normalized_f64(const double* Y, double* X, const int32_t* source_nodes,
-                                 double alpha, int64_t N, int64_t K, double* errors, void* ws,
-                                 int64_t ws_bytes, void* stream) {
-  GL_REQUIRE(N >= 1 && K >= 1, "bad sizes (N=%lld K=%lld)", (long long)N, (long long)K);
-  GL_REQUIRE(Y && X && source_nodes && errors, "null pointer");
-  int64_t nb, rpb;
-  update_shape(N, K, nb, rpb);
-  GL_REQUIRE(ws != nullptr && ws_bytes >= nb * K * 8, "workspace too small: %lld < %lld bytes",
-             (long long)ws_bytes, (long long)(nb * K * 8));
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_ppr_update_f64, dim3((unsigned)nb), dim3(256), 0, st, Y, X,
-                     source_nodes, alpha, N, K, rpb, (double*)ws);
-  GL_LAUNCH_CHECK("k_ppr_update_f64");
-  hipLaunchKernelGGL(k_ppr_err_f64, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st,
-                     (const double*)ws, nb, K, errors);
-  GL_LAUNCH_CHECK("k_ppr_err_f64");
   return GL_OK;
 }
 

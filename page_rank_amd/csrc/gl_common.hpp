@@ -23,10 +23,23 @@ struct DeviceInfo {
 // Cached per current device.
 const DeviceInfo& device_info();
 
-// Side stream used for the CSR-remainder launch in split mode and for graph capture.
-// Created once per device, default priority, non-blocking.
-hipStream_t side_stream();
-hipStream_t capture_stream();
+// A non-blocking stream (default priority) with two events, leased by one call at a time.
+struct StreamSet {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr;   // recorded on the caller's stream, waited on by `stream`
+  hipEvent_t ev_out = nullptr;  // recorded on `stream`, waited on by the caller's stream
+};
+// Lease a set of the current device (nullptr and the last error set on failure).
+StreamSet* acquire_stream_set();
+void release_stream_set(StreamSet* s);
+struct StreamLease {
+  StreamSet* s;
+  StreamLease() : s(acquire_stream_set()) {}
+  ~StreamLease() { release_stream_set(s); }
+  StreamLease(const StreamLease&) = delete;
+  StreamLease& operator=(const StreamLease&) = delete;
+};
 
 }  // namespace gl
 

@@ -27,7 +27,6 @@
 
 namespace gl {
 
-hipEvent_t lib_event(int i);
 // Persistent grid: enough 4-wave workgroups for `work_items` waves, capped at wg_per_cu
 // per CU (0 = 8), rounded to a multiple of the XCD count.
 int grid_for(int64_t work_items, int wg_per_cu);

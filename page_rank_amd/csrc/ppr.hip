@@ -13,7 +13,6 @@
 namespace gl {
 
 constexpr int kMaxWg = 4096;      // cap of persistent workgroups (partials rows)
-constexpr int kMaxK = 1024;       // seed sets per run
 constexpr int kRetireThreads = 1024;
 
 __global__ void k_fill(float* __restrict__ x, int64_t n, float v) {
@@ -126,13 +125,14 @@ __global__ void k_ctl_init(PprCtl* ctl, ColState c, int64_t K, float alpha, int6
 // max_iter, and refresh the dangling coefficients for the next pass. One workgroup:
 // consecutive threads take consecutive columns (coalesced rows of the partial arrays) and
 // kRetireThreads / KC slices of threads split the workgroup rows; an LDS tree in fixed order
-// combines the slices, so the sums are deterministic.
+// combines the slices, so the sums are deterministic. The column residuals go through
+// `res` (K floats of the workspace; the barriers order those global writes and reads within
+// the workgroup), so K has no LDS-imposed limit.
 constexpr int kUnroll = 8;
 __global__ void __launch_bounds__(kRetireThreads)
 k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ part_res,
-         const float* __restrict__ part_dang, float* __restrict__ hist, float tol, float alpha,
-         int64_t N, int dmode, int together, int max_iter) {
-  __shared__ float s_res[kMaxK];
+         const float* __restrict__ part_dang, float* __restrict__ res, float* __restrict__ hist,
+         float tol, float alpha, int64_t N, int dmode, int together, int max_iter) {
   __shared__ double s_r[kRetireThreads];
   __shared__ double s_d[kRetireThreads];
   __shared__ int s_max_bits;
@@ -188,7 +188,7 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
     }
     if (sl == 0 && k < K) {
       const float rf = (float)s_r[kk];
-      s_res[k] = rf;
+      res[k] = rf;  // read back below by other threads of this workgroup
       if (!c.frozen[k]) {
         c.dsum[k] = (float)s_d[kk];
         atomicMax(&s_max_bits, __float_as_int(rf));  // rf >= 0: int order == float order
@@ -203,7 +203,7 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
       if (hist) hist[(int64_t)it * K + k] = 0.f;
       continue;
     }
-    const float r = s_res[k];
+    const float r = res[k];
     if (hist) hist[(int64_t)it * K + k] = r;
     const bool conv = together ? (maxres < tol) : (r < tol);
     if (conv || last) {
@@ -231,6 +231,7 @@ struct Workspace {
   float* dsum;
   float* cu;
   float* tele;
+  float* res;
   float* part_res;
   float* part_dang;
   int32_t* sptr;
@@ -255,6 +256,7 @@ static Workspace carve(char* base, int64_t rows, int32_t B, int64_t K, int64_t E
   w.dsum = (float*)take(K * 4);
   w.cu = (float*)take(K * 4);
   w.tele = (float*)take(K * 4);
+  w.res = (float*)take(K * 4);
   w.part_res = (float*)take((int64_t)kMaxWg * K * 4);
   w.part_dang = (float*)take((int64_t)kMaxWg * K * 4);
   w.sptr = (int32_t*)take((ns + 1) * 4);
@@ -301,12 +303,12 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
              "Unsupported block size %dx%d: only 2x2, 4x4 and 8x8 are supported", block_rows,
              block_cols);
   GL_REQUIRE(rows <= INT32_MAX, "rows exceed int32 indices");
-  GL_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, %d]", kMaxK);
+  GL_REQUIRE(K >= 1 && K <= INT32_MAX, "K must be in [1, 2^31)");
   GL_REQUIRE(seed_ptr && seed_idx, "seed arrays are null");
   GL_REQUIRE(X_out && iterations && converged, "null output");
   GL_REQUIRE(stripe_ptr && rem_indptr, "null stripe_ptr / remainder indptr");
   GL_REQUIRE(num_tiles == 0 || (codes && coords && offsets && data), "null tile arrays");
-  GL_REQUIRE(opts->max_iter >= 1, "max_iter must be >= 1");
+  GL_REQUIRE(opts->max_iter >= 0, "max_iter must be >= 0");
   GL_REQUIRE(opts->dangling_mode >= 0 && opts->dangling_mode <= 2, "bad dangling_mode");
   GL_REQUIRE(opts->dangling_mode == GL_DANGLING_NONE || dangling_mask != nullptr,
              "dangling_mask required for dangling_mode %d", opts->dangling_mode);
@@ -356,10 +358,14 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   Workspace w = carve((char*)workspace, rows, block_rows, K, E);
   const int64_t ns = (rows + block_rows - 1) / block_rows;
 
+  // the run's own stream (graph capture needs one; the caller's may be the null stream),
+  // leased for this call only, so concurrent runs never share it
+  StreamLease lease;
+  if (!lease.s) return GL_ERR_HIP;
   hipStream_t user = (hipStream_t)stream;
-  hipStream_t st = capture_stream();
-  GL_HIP(hipEventRecord(lib_event(2), user));
-  GL_HIP(hipStreamWaitEvent(st, lib_event(2), 0));
+  hipStream_t st = lease.s->stream;
+  GL_HIP(hipEventRecord(lease.s->ev_in, user));
+  GL_HIP(hipStreamWaitEvent(st, lease.s->ev_in, 0));
 
   GL_HIP(hipMemcpyAsync(w.srow, hrow.data(), E * 4, hipMemcpyHostToDevice, st));
   GL_HIP(hipMemcpyAsync(w.scol, hcol.data(), E * 4, hipMemcpyHostToDevice, st));
@@ -368,10 +374,10 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
                      ns, block_rows, w.srow, E);
   GL_LAUNCH_CHECK("k_seed_stripes");
 
-  // ---- start vector in X_out (buffer 0) ----
+  // ---- start vector in X_out (buffer 0); X0 == X_out is a warm start in place ----
   const int64_t NK = rows * K;
-  if (X0 && X0 != X_out) {
-    GL_HIP(hipMemcpyAsync(X_out, X0, NK * 4, hipMemcpyDeviceToDevice, st));
+  if (X0) {
+    if (X0 != X_out) GL_HIP(hipMemcpyAsync(X_out, X0, NK * 4, hipMemcpyDeviceToDevice, st));
   } else {
     hipLaunchKernelGGL(k_fill, dim3(blocks_for(NK, 256)), dim3(256), 0, st, X_out, NK, 0.f);
     GL_LAUNCH_CHECK("k_fill");
@@ -395,10 +401,18 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1024), 0, st, w.ctl, cs, K, opts->alpha, rows,
                      opts->dangling_mode);
   GL_LAUNCH_CHECK("k_ctl_init");
-  if (history) {
+  if (history && opts->max_iter > 0) {
     hipLaunchKernelGGL(k_fill, dim3(blocks_for((int64_t)opts->max_iter * K, 256)), dim3(256),
                        0, st, history, (int64_t)opts->max_iter * K, 0.f);
     GL_LAUNCH_CHECK("k_fill");
+  }
+  if (opts->max_iter == 0) {
+    // no pass: X_out holds the start vector, every column 0 iterations, not converged
+    // (the reference's loop body never runs and returns (X, 0, False))
+    if (total_iterations) *total_iterations = 0;
+    GL_HIP(hipEventRecord(lease.s->ev_out, st));
+    GL_HIP(hipStreamWaitEvent(user, lease.s->ev_out, 0));
+    return GL_OK;
   }
 
   PbrView m{rows, cols, ns, num_tiles, num_vals, num_rem, codes, coords, offsets, data,
@@ -431,7 +445,7 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
       if (rc) return rc;
     }
     hipLaunchKernelGGL(k_retire, dim3(1), dim3(kRetireThreads), 0, st, w.ctl, cs, K, nwg,
-                       w.part_res, w.part_dang, history, opts->tol, opts->alpha, rows,
+                       w.part_res, w.part_dang, w.res, history, opts->tol, opts->alpha, rows,
                        opts->dangling_mode, together, opts->max_iter);
     GL_LAUNCH_CHECK("k_retire");
     return GL_OK;
@@ -480,8 +494,8 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   if (done_iters % 2 == 1)
     GL_HIP(hipMemcpyAsync(X_out, w.xb, NK * 4, hipMemcpyDeviceToDevice, st));
   if (total_iterations) *total_iterations = done_iters;
-  GL_HIP(hipEventRecord(lib_event(3), st));
-  GL_HIP(hipStreamWaitEvent(user, lib_event(3), 0));
+  GL_HIP(hipEventRecord(lease.s->ev_out, st));
+  GL_HIP(hipStreamWaitEvent(user, lease.s->ev_out, 0));
   return GL_OK;
 }
 

@@ -2,6 +2,7 @@
 (Backend/graph_link/graph_link/__init__.py:66-134)."""
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _native
@@ -84,13 +85,13 @@ def pbr_batched_matmul_cuda(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0) -> 
 
 
 def pbr_matmul(pbr_mat, x: torch.Tensor, *, mode="fused", kw=0):
-    """Unified entry point: x on the GPU -> HIP SpMM (uploading the matrix on first use)."""
+    """Unified entry point (__init__.py:104-134): x on the GPU -> HIP SpMM (uploading the
+    matrix on first use); x on the CPU -> the host SpMM of the library."""
     if x.dim() not in (2, 3):
         raise ValueError(f"Expected 2D or 3D input, got {x.dim()}D")
     if not x.is_cuda:
-        raise NotImplementedError(
-            "pbr_matmul: this build runs on the GPU only; move x to a HIP device "
-            "(the reference's CPU SpMM is not part of it)")
+        y = pbr_batched_matmul_cpu(pbr_mat, x)
+        return y.squeeze(0) if x.dim() == 2 else y
     if get_pbr_gpu_meta(pbr_mat) is None:
         pbr_mat = pbr_mat.to(x.device)
     y = pbr_batched_matmul_cuda(pbr_mat, x, mode=mode, kw=kw)
@@ -136,9 +137,42 @@ def pbr_batched_matmul_triton_gpu(pbr_mat, x: torch.Tensor, y: torch.Tensor, bat
     return y
 
 
-def pbr_batched_matmul_cpu(pbr_mat, x):
-    raise NotImplementedError("pbr_batched_matmul_cpu: this build has no CPU SpMM; use "
-                              "pbr_matmul with a GPU tensor")
+def pbr_batched_matmul_cpu(pbr_mat, x: torch.Tensor, *, threads: int = 0) -> torch.Tensor:
+    """Y = A @ x on the host, shape [batch, rows, features] (x is a CPU tensor [cols, K] or
+    [batch, cols, K] of the matrix's value type), as the reference's pbr_batched_matmul_cpu
+    (Backend/graph_link/pbr_matrix/pbr_matrix.cpp:17-68, bound at bindings.cpp:65-68).
+    Runs gl_pbr_spmm_host_* of the native library (bit-identical to the reference's loop)
+    on `threads` host threads (0 = all cores)."""
+    if x.dim() not in (2, 3):
+        raise ValueError(f"Expected 2D or 3D input, got {x.dim()}D")
+    if x.is_cuda:
+        raise ValueError("pbr_batched_matmul_cpu needs a CPU tensor")
+    st = np.dtype(pbr_mat.scalar_dtype)
+    it = np.dtype(pbr_mat.index_dtype)
+    tdt = torch.float64 if st == np.float64 else torch.float32
+    if x.dtype != tdt:
+        raise TypeError(f"x is {x.dtype}, the matrix holds {st}")
+    x3 = (x.unsqueeze(0) if x.dim() == 2 else x).contiguous()
+    if x3.shape[1] != pbr_mat.cols:
+        raise ValueError(f"shape mismatch: A is {pbr_mat.rows}x{pbr_mat.cols}, x "
+                         f"{tuple(x.shape)}")
+    B, _, K = x3.shape
+    y = torch.empty((B, pbr_mat.rows, K), dtype=tdt)
+    if K == 0 or B == 0:
+        return y.zero_()
+    L = _native.lib()
+    fn = getattr(L, f"gl_pbr_spmm_host_{'i64' if it == np.int64 else 'i32'}_"
+                    f"{'f64' if st == np.float64 else 'f32'}")
+    arrs = [np.ascontiguousarray(a) for a in (
+        pbr_mat.block_codes, pbr_mat.block_coords, pbr_mat.block_offsets, pbr_mat.block_data,
+        pbr_mat.remainder_indptr, pbr_mat.remainder_col_ind, pbr_mat.remainder_data)]
+    for b in range(B):
+        xb, yb = x3[b], y[b]
+        rc = fn(pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
+                arrs[0].size, *(a.ctypes.data for a in arrs), xb.data_ptr(), K, K,
+                yb.data_ptr(), K, int(threads))
+        _native.check(rc, "gl_pbr_spmm_host")
+    return y
 
 
 class SpmmPlan:

@@ -60,10 +60,21 @@ class PBRMatrix:
         self._host = None
         if host is not None:
             self._host = {k: _ro(host[k]) for k in _HOST_FIELDS}
-        self._lazy_meta = None
+        self._dev = None  # device arrays (dict of tensors) while resident on a GPU
         if device_meta is not None:
             set_pbr_gpu_meta(self, device_meta)
-            self._lazy_meta = device_meta
+
+    # ---- device residency (the registry functions read and write these) ----
+    def device_arrays(self):
+        return self._dev
+
+    def attach_device_arrays(self, meta):
+        self._dev = meta
+
+    def detach_device_arrays(self):
+        if self._dev is not None and self._host is None:
+            self._h()  # a GPU-built matrix keeps its arrays on the host
+        self._dev = None
 
     # ---- read-only scalars (bindings.cpp:11-16) ----
     rows = property(lambda self: self._rows)
@@ -76,7 +87,7 @@ class PBRMatrix:
     # ---- arrays (bindings.cpp:17-46) ----
     def _h(self):
         if self._host is None:
-            meta = self._lazy_meta
+            meta = self._dev
             if meta is None:
                 raise RuntimeError("PBR matrix has neither host nor device arrays")
             it, st = self.index_dtype, self.scalar_dtype
@@ -101,26 +112,27 @@ class PBRMatrix:
 
     # ---- counts (pbr_matrix.hpp:156-166) ----
     def accounted_blocks(self) -> int:
-        if self._host is None and self._lazy_meta is not None:
-            return int(self._lazy_meta["codes"].numel())
+        if self._host is None and self._dev is not None:
+            return int(self._dev["codes"].numel())
         return int(self._h()["block_codes"].size)
 
     def compressed_nnz(self) -> int:
-        if self._host is None and self._lazy_meta is not None:
-            return int(self._lazy_meta["data"].numel())
+        if self._host is None and self._dev is not None:
+            return int(self._dev["data"].numel())
         return int(self._h()["block_data"].size)
 
     def remainder_nnz(self) -> int:
-        if self._host is None and self._lazy_meta is not None:
-            return int(self._lazy_meta["rem_col_ind"].numel())
+        if self._host is None and self._dev is not None:
+            return int(self._dev["rem_col_ind"].numel())
         return int(self._h()["remainder_col_ind"].size)
 
     def __copy__(self):
+        """A new matrix with the same host arrays and no device arrays (the reference's copy
+        constructs a new C++ object, which its registry does not know)."""
         new = object.__new__(type(self))
         new.__dict__.update(self.__dict__)
-        new._lazy_meta = None
-        if self._host is None:
-            new._host = self._h()
+        new._host = self._h()
+        new._dev = None
         return new
 
     def __repr__(self):

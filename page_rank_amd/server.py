@@ -8,13 +8,19 @@ POST /api/ppr     {"graph": {...}, "params": {...}} ->
                   {"ranks", "iterations", "converged", "convergenceHistory", "degrees"}
 
 The request and response schemas, the 503 (no GPU) and 422 (no nodes / no valid seed)
-answers and the degree counts are the reference's. What differs is inside:
-- the seed set is one personalization column (weights 1/|S|, duplicates add up) instead of
-  the mean of one column per seed, which is the same vector by linearity;
-- normalization runs on the GPU with dangling columns handled inside the run (uniform), not
-  densified, so large graphs with many dangling nodes fit;
+answers and the degree counts are the reference's, and so is the computation's shape: one
+column per seed entry (duplicates included), every column stopping when the largest column
+residual falls below the tolerance, ranks = the column mean renormalized to sum 1,
+`iterations` = passes run (maxIter when not converged) and `convergenceHistory[t]` = the
+largest column residual of pass t. What differs is inside:
+- normalization runs on the GPU with dangling columns handled inside the run (their mass
+  spread uniformly, as the reference's dense 1/N columns do), so large graphs with many
+  dangling nodes fit;
+- the matrix is tiled 8x8 (the reference's default is 2x2), which changes no result beyond
+  float32 rounding;
 - the whole loop is one hipGraph-driven call.
-POST /api/ppr/batch takes several seed sets and answers one rank map per set (an addition).
+POST /api/ppr/batch takes several seed sets and answers one rank map per set (an addition):
+each set is one personalization column (weights 1/|S|) that stops at its own residual.
 """
 from __future__ import annotations
 
@@ -104,14 +110,14 @@ def _prepare(g: Graph):
     return ids, index_of, A, {"in": in_deg, "out": out_deg}
 
 
-def _solve(A, seed_sets, p: PPRParams):
+def _solve(A, seed_sets, p: PPRParams, retire):
     import page_rank_amd as gl
     dev = torch.device("cuda", torch.cuda.current_device())
     norm, dang = gl.normalize_transition_matrix_gpu(A, dev)
     pbr = gl.csr_to_pbr(norm, 8, 8, 4, order="canonical")
-    return gl.run_ppr(pbr, seed_sets, alpha=p.alpha, tol=p.tolerance, max_iter=max(1, p.maxIter),
+    return gl.run_ppr(pbr, seed_sets, alpha=p.alpha, tol=p.tolerance, max_iter=max(0, p.maxIter),
                       dangling=dang if dang.any() else None,
-                      dangling_mode="uniform" if dang.any() else "none", retire="together")
+                      dangling_mode="uniform" if dang.any() else "none", retire=retire)
 
 
 def _ranks(x: np.ndarray, ids):
@@ -133,12 +139,19 @@ def run_ppr_endpoint(req: PPRRequest) -> dict:
     if not seed_idx:
         raise HTTPException(status_code=422,
                             detail="Personalized PageRank requires at least one seed node.")
-    res = _solve(A, [seed_idx], p)
-    x = res.X[:, 0].detach().cpu().numpy().astype(np.float64)
+    # one column per seed entry, stopping together (app.py:132-152)
+    res = _solve(A, [[i] for i in seed_idx], p, "together")
+    r = res.X.mean(dim=1)
+    total = float(r.sum().item())
+    if total > 0:
+        r = r / total
+    r = r.detach().cpu().numpy()
     T = res.total_iterations
-    return {"ranks": _ranks(x, ids), "iterations": int(T),
-            "converged": bool(res.converged[0]),
-            "convergenceHistory": [float(h) for h in res.history[:T, 0]],
+    history = res.history[:T].max(axis=1) if T > 0 else np.zeros(0, np.float32)
+    return {"ranks": {ids[i]: float(r[i]) for i in range(len(ids))},
+            "iterations": int(T) if p.maxIter > 0 else int(p.maxIter),
+            "converged": bool(res.converged.all()),
+            "convergenceHistory": [float(h) for h in history],
             "degrees": degrees}
 
 
@@ -158,7 +171,7 @@ def run_ppr_batch(req: PPRBatchRequest) -> dict:
         if not idx:
             raise HTTPException(status_code=422, detail=f"Seed set {k} has no valid seed node.")
         sets.append(idx)
-    res = _solve(A, sets, p)
+    res = _solve(A, sets, p, "per_column")
     X = res.X.detach().cpu().numpy().astype(np.float64)
     return {"results": [{"ranks": _ranks(X[:, k], ids), "iterations": int(res.iterations[k]),
                          "converged": bool(res.converged[k])} for k in range(len(sets))],

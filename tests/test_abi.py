@@ -40,7 +40,7 @@ def test_ctypes_table_matches_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 4
+    assert L.gl_abi_version() == 5
     assert L.gl_last_error() is not None
 
 
@@ -82,4 +82,28 @@ def test_fp64_argument_validation_without_device():
     assert L.gl_ppr_update_f64_workspace_bytes(0, 1) == -1
     assert L.gl_ppr_update_f64_workspace_bytes(1000, 3) >= 3 * 8
     rc = L.gl_ppr_update_normalized_f64(1, 1, 1, 0.85, 1000, 3, 1, None, 0, None)
-    assert rc == _native.GL_ERR_INVALID and "workspace too small" in _native.last_error()
+    assert rc == _native.GL_ERR_WORKSPACE and "workspace too small" in _native.last_error()
+
+
+def test_fp32_step_argument_validation_without_device():
+    L = _native.lib()
+    assert L.gl_ppr_init_f32(None, None, 0, 1, None) == _native.GL_ERR_INVALID
+    assert L.gl_ppr_update_f32_workspace_bytes(0, 1) == -1
+    assert L.gl_ppr_update_f32_workspace_bytes(1000, 3) == \
+        L.gl_ppr_update_f64_workspace_bytes(1000, 3) // 2
+    rc = L.gl_ppr_update_normalized_f32(1, 1, 1, 0.85, 1000, 3, 1, None, 0, None)
+    assert rc == _native.GL_ERR_WORKSPACE and "workspace too small" in _native.last_error()
+    rc = L.gl_ppr_missing_mass_f32(None, 1, 10, 2, None, 0, None)
+    assert rc == _native.GL_ERR_INVALID and "null" in _native.last_error()
+    rc = L.gl_ppr_update_f32(8, 8, 8, 8, 8, 0.85, 10, 2, 8, None, 0, None)
+    assert rc == _native.GL_ERR_INVALID and "different buffers" in _native.last_error()
+    rc = L.gl_pbr_to_csr_i32_f64(4, 4, 2, 2, 0, 0, 0, *([None] * 10), 0, None, 0, None)
+    assert rc == _native.GL_ERR_INVALID
+
+
+def test_host_spmm_argument_validation():
+    L = _native.lib()
+    rc = L.gl_pbr_spmm_host_i32_f32(4, 4, 9, 9, 0, *([None] * 7), None, 1, 1, None, 1, 1)
+    assert rc == _native.GL_ERR_INVALID and "64-bit code" in _native.last_error()
+    rc = L.gl_pbr_spmm_host_i64_f64(4, 4, 2, 2, 0, *([None] * 7), None, 1, 2, None, 2, 1)
+    assert rc == _native.GL_ERR_INVALID and "leading dimension" in _native.last_error()

@@ -81,6 +81,13 @@ def test_karate_ranks_match_fp64(device):
     got = np.array([out["ranks"][str(i)] for i in nodes])
     assert abs(got.sum() - 1) < 1e-6
     assert np.abs(got - x[:, 0]).max() < 5e-6
+    # iterations / history as the reference computes them (app.py:132-152): one column per
+    # seed entry, all stopping when the largest column residual is below the tolerance
+    cols = oppr.personalization_matrix(35, [[0], [33], [0]])
+    _, it_o, _, hist_o = oppr.power(An, cols, tol=1e-7, max_iter=200, dangling=dang,
+                                    dangling_mode="uniform", retire="together")
+    assert abs(out["iterations"] - int(it_o.max())) <= 1
+    np.testing.assert_allclose(out["convergenceHistory"][:5], hist_o[:5].max(axis=1), rtol=1e-3)
     assert out["degrees"]["out"]["33"] == 1.0 and out["degrees"]["in"]["34"] == 1.0
     rb = client.post("/api/ppr/batch", json={**body, "seedSets": [["0"], ["33", "1"]]})
     assert rb.status_code == 200 and len(rb.json()["results"]) == 2
