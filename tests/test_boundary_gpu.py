@@ -111,7 +111,7 @@ def test_more_than_1024_sources(device):
     n = 4096
     A = graph_link.normalize_transition_matrix(_er_graph(n, 5, 11))
     pbr = graph_link.csr_to_pbr(A, 8, 8, 2).to(device)
-    sources = np.arange(0, n, 4)[:1025]
+    sources = np.arange(0, n, 3)[:1025]
     X, iters, conv = graph_link.run_personalized_pagerank(
         pbr, torch.tensor(sources, dtype=torch.int32, device=device), tolerance=1e-6,
         max_iterations=200)
