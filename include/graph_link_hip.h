@@ -341,6 +341,18 @@ int gl_ppr_update_normalized_f64(const double* Y, double* X, const int32_t* sour
                                  double alpha, int64_t N, int64_t K, double* errors,
                                  void* workspace, int64_t workspace_bytes, void* stream);
 
+/* gl_ppr_update_normalized_f32 without a workspace: the per-block partial sums live in a
+ * static device array of the library (4 MiB on every GPU it runs on, held until the process
+ * ends), so the call allocates nothing. No counterpart in the reference (whose update needs
+ * no workspace, but whose errors are float atomics): it is for callers that cannot allocate,
+ * such as code under HIP graph capture and tight serving loops. Limit: N * K small enough
+ * that the partials fit 4 MiB (any N for K <= 1024); larger sizes return GL_ERR_INVALID.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (several
+ * streams or threads at once) race; calls in sequence on one stream are safe. */
+int gl_ppr_update_normalized_f32_noalloc(const float* Y, float* X, const int32_t* source_nodes,
+                                         float alpha, int64_t N, int64_t K, float* errors,
+                                         void* stream);
+
 /* col_sums[k] = sum_i Y[i][k] (overwritten). Replaces missing_mass_cuda_float / _double
  * (ppr_kernel.cu:13-48, bound at bindings_kernels.cu:152-153). */
 int gl_ppr_missing_mass_f32(const float* Y, float* col_sums, int64_t N, int64_t K,

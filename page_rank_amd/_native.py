@@ -120,6 +120,8 @@ _PROTOS = {
                                                     c_i64, c_p, c_p, c_i64, c_p]),
     "gl_ppr_update_normalized_f64": (ctypes.c_int, [c_p, c_p, c_p, ctypes.c_double, c_i64,
                                                     c_i64, c_p, c_p, c_i64, c_p]),
+    "gl_ppr_update_normalized_f32_noalloc": (ctypes.c_int, [c_p, c_p, c_p, c_f32, c_i64, c_i64,
+                                                            c_p, c_p]),
     "gl_ppr_missing_mass_f32": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p, c_i64, c_p]),
     "gl_ppr_missing_mass_f64": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_p, c_i64, c_p]),
     "gl_ppr_update_f32": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_p, c_f32, c_i64, c_i64, c_p,

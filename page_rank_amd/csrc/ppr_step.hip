@@ -11,10 +11,50 @@
 // order) and a second kernel adds the partials in block order, so results do not depend on
 // scheduling.
 #include <algorithm>
+#include <mutex>
 
 #include "gl_common.hpp"
 
 namespace gl {
+
+// Scratch of gl_ppr_update_normalized_f32_noalloc: the per-block partial sums that the
+// workspace holds for gl_ppr_update_normalized_f32. A file-scope device array, so the call
+// allocates nothing; every GPU the library runs on holds its own copy (kScratchBytes) until
+// the process ends. In host code the array's name is not a device address: its address on
+// each device comes from hipGetSymbolAddress, taken once per device.
+constexpr int64_t kScratchBytes = 4 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (nullptr and the last error set on
+// failure).
+static void* scratch_ptr() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    hip_fail(e, "hipGetDevice", __FILE__, __LINE__);
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d is beyond the %d devices the scratch table covers", dev, kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)",
+               __FILE__, __LINE__);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
 
 // X = one-hot columns: X[i][k] = (i == src[k]). The reference's init only sets the ones
 // into a zeroed X (ppr_kernel.cu:199-216); this writes every element, so X need not be
@@ -215,6 +255,25 @@ int gl_ppr_update_normalized_f32(const float* Y, float* X, const int32_t* source
                                  int64_t ws_bytes, void* stream) {
   return ppr_update_normalized<float>(Y, X, source_nodes, alpha, N, K, errors, ws, ws_bytes,
                                       stream);
+}
+
+int gl_ppr_update_normalized_f32_noalloc(const float* Y, float* X, const int32_t* source_nodes,
+                                         float alpha, int64_t N, int64_t K, float* errors,
+                                         void* stream) {
+  GL_REQUIRE(N >= 1 && K >= 1, "bad sizes (N=%lld K=%lld)", (long long)N, (long long)K);
+  GL_REQUIRE(Y && X && source_nodes && errors, "null pointer");
+  const int64_t need = cols_workspace_bytes<float>(N, K);
+  GL_REQUIRE(need <= kScratchBytes,
+             "N=%lld, K=%lld needs %lld bytes of partial sums, above the %lld-byte static "
+             "scratch of this entry point (any N works for K <= 1024); use "
+             "gl_ppr_update_normalized_f32 with a workspace",
+             (long long)N, (long long)K, (long long)need, (long long)kScratchBytes);
+  void* part = scratch_ptr();
+  if (!part) return GL_ERR_HIP;
+  // every partial that the sum reads is written by this call's first kernel: nothing of an
+  // earlier call is read, so the scratch needs no clearing
+  return ppr_cols<float, 0>(Y, X, X, source_nodes, nullptr, alpha, N, K, errors, 0,
+                            (float*)part, (hipStream_t)stream);
 }
 
 int gl_ppr_update_normalized_f64(const double* Y, double* X, const int32_t* source_nodes,
