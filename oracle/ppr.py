@@ -107,10 +107,12 @@ def power(A, P, alpha=0.85, tol=1e-6, max_iter=100, dangling=None, dangling_mode
     return X, iters, conv, np.array(hist)
 
 
-def nx_pagerank(A_col, p, alpha=0.85, tol=1e-6, max_iter=100, dangling_weights=None):
+def nx_pagerank(A_col, p, alpha=0.85, tol=1e-6, max_iter=100, dangling_weights=None,
+                return_iterations=False):
     """networkx _pagerank_scipy semantics on a column-oriented weighted adjacency A_col
-    (A_col[target, source] = weight, any non-negative weights). Returns x normalised to 1.
-    Raises RuntimeError (networkx: PowerIterationFailedConvergence) after max_iter."""
+    (A_col[target, source] = weight, any non-negative weights). Returns x normalised to 1
+    (and the iteration count with return_iterations). Raises RuntimeError (networkx:
+    PowerIterationFailedConvergence) after max_iter."""
     A_col = sp.csc_matrix(A_col, dtype=np.float64)
     n = A_col.shape[0]
     S = np.asarray(A_col.sum(axis=0)).ravel()         # out-weight of each source
@@ -122,12 +124,12 @@ def nx_pagerank(A_col, p, alpha=0.85, tol=1e-6, max_iter=100, dangling_weights=N
     dw = p if dangling_weights is None else np.asarray(dangling_weights, np.float64)
     dw = dw / dw.sum()
     is_dangling = S == 0
-    for _ in range(max_iter):
+    for it in range(max_iter):
         xlast = x
         x = alpha * (M @ x + x[is_dangling].sum() * dw) + (1 - alpha) * p
         err = np.abs(x - xlast).sum()
         if err < n * tol:
-            return x / x.sum()
+            return (x / x.sum(), it + 1) if return_iterations else x / x.sum()
     raise RuntimeError(f"power iteration failed to converge within {max_iter} iterations")
 
 

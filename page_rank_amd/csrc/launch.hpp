@@ -2,28 +2,11 @@
 #pragma once
 
 #include "gl_common.hpp"
-#include "mfma_kernels.hpp"
 #include "stripe_kernels.hpp"
 
 #include <algorithm>
 #include <atomic>
 #include <type_traits>
-
-// Matrix-core tile product for 32 / 64 columns of densely tiled 8x8 SpMMs (1 = on).
-// Measured on MI355X (1M x 1M, 8 tiles per stripe): 10 % slower at 32 columns (block part
-// 202 vs 181 us) and 4 % faster at 64 (494 vs 516 us). Pairing two 8-row stripes in one
-// 16x16x4 product leaves half of every MFMA multiplying zeros, so the matrix pipe does no
-// more useful work per cycle than the VALU FMAs it replaces. Off.
-#ifndef GL_MFMA
-#define GL_MFMA 0
-#endif
-// VEC kernels for buffer-descriptor launches: 0 = full, 16-byte aligned column chunks only;
-// 1 = partial chunks too; 2 = any row pitch as well (dword-aligned vector loads). Measured on
-// MI355X (1M x 1M, 8x8) against 0: K=3 198 -> 125 us, K=6 306 -> 164 us, K=20 249 -> 207 us,
-// K=24 251 -> 218 us, K=48 487 -> 345 us; full aligned chunks unchanged.
-#ifndef GL_VEC_PARTIAL
-#define GL_VEC_PARTIAL 2
-#endif
 
 namespace gl {
 
@@ -91,23 +74,13 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
   // feature-lane kernels that is 8 / (64 / KW) groups, 2 at 16 columns (8 columns computed
   // with kw 16: 234 -> 196 us against 4 groups) and 4 at 32; 64 columns stay at 4 (registers).
   const bool wide = B == 8 && m.num_tiles >= 6 * m.num_stripes;
-  if constexpr (GL_MFMA && B == 8 && TILES && std::is_same<Epi, EpiStore>::value) {
-    if (wide && (kw == 32 || kw == 64)) {
-      if (kw == 32) GL_GO((k_stripe_mfma<32, REM, BUF>));
-      else GL_GO((k_stripe_mfma<64, REM, BUF>));
-      if (!occ) GL_LAUNCH_CHECK("k_stripe_mfma");
-      return GL_OK;
-    }
-  }
   // float4 X-row loads (the VEC kernels, no runtime branch per X load). The plain-pointer
   // form needs 16-byte aligned rows and a full column chunk. Buffer loads need only dword
   // alignment and are range-checked per dword, so there a partial last chunk reads the
   // columns past K as well: the next row's values (0 past the array), which only reach
   // accumulators of columns that are not stored.
   const bool full = std::min<int64_t>(kw, d.K - d.k0) == kw;
-  const bool vec = GL_VEC_PARTIAL == 2 && BUF
-                       ? true
-                       : (d.ldx & 3) == 0 && (full || (GL_VEC_PARTIAL >= 1 && BUF));
+  const bool vec = BUF || ((d.ldx & 3) == 0 && full);
   switch (kw) {
 #define GL_KB(KW)                                                                        \
     case KW:                                                                             \

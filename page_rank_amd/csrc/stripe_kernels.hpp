@@ -22,45 +22,6 @@
 
 #include "gl_common.hpp"
 
-#ifndef GL_FUSED_DPP_FMA
-#define GL_FUSED_DPP_FMA 1
-#endif
-// Bit-lane kernel: issue the loads of the next tile batch (and, at a stripe's last batch,
-// of the next stripe's first batch) before the FMAs of the current one. Measured on MI355X:
-// 3-12 % slower for SpMM and 48 % slower for the 8-column PPR step, so it is off.
-#ifndef GL_BIT_PIPE
-#define GL_BIT_PIPE 0
-#endif
-// Densely tiled 8x8 matrices (the launcher's 8-tile batch) at one RHS column: lane (row r,
-// slot q) walks the 8 bits of row r of tile q of each 8-tile pass (1), instead of one lane
-// per tile bit (0). Measured on MI355X (1M x 1M, 8 tiles per stripe): tiles 76 vs 93 us, the
-// fused SpMM 113.5 vs 128.5 us. On sparse tilings most of the 8 slots would stay empty.
-#ifndef GL_BIT_ROWLANE
-#define GL_BIT_ROWLANE 1
-#endif
-// Cache policy of the loads that stream the matrix (each byte read once per pass): 0 =
-// default, 2 = NT (non-temporal). Measured on MI355X, NT was 3-13 % slower (SpMM and the
-// PPR step), so the default policy stays.
-#ifndef GL_STREAM_CP
-#define GL_STREAM_CP 0
-#endif
-// Feature-lane kernel: issue the loads of the next tile batch (and, at a stripe's last
-// batch, of the next stripe's first batch) before the FMAs of the current one. Measured on
-// MI355X: no gain for SpMM, and 25 % slower for the K=64 PPR step (the second batch of
-// registers costs a wave per SIMD), so it is off.
-#ifndef GL_FEAT_PIPE
-#define GL_FEAT_PIPE 0
-#endif
-// Feature-lane remainder: split a stripe's remainder entries (contiguous in the CSR) evenly
-// over lane groups instead of walking its rows in lockstep (1), or row lanes (0).
-// Measured on MI355X: 28 % slower for SpMM at 32 columns and 18 % for the 64-column PPR step
-// (the LDS flushes and row lookups cost more than the balance gains), so it is off.
-#ifndef GL_REM_FLAT
-#define GL_REM_FLAT 0
-#endif
-#ifndef GL_REM_FLAT_EB
-#define GL_REM_FLAT_EB 4
-#endif
 // Remainder entries whose X-row gathers are issued together (feature-lane kernel).
 #ifndef GL_REM_EB
 #define GL_REM_EB 4
@@ -76,33 +37,10 @@
 #ifndef GL_REM_EB64_PTR
 #define GL_REM_EB64_PTR 2
 #endif
-// Minimum waves per SIMD the feature-lane kernel is compiled for (caps its VGPRs); 0 = free.
-#ifndef GL_FEAT_WAVES
-#define GL_FEAT_WAVES 0
-#endif
 // Minimum waves per SIMD of the 64-column feature-lane kernel (caps its VGPRs).
 #ifndef GL_FEAT64_WAVES
 #define GL_FEAT64_WAVES 4
 #endif
-// Feature-lane kernels on 8x8 tiles with 32 or 64 columns: the DPP rows that share a tile
-// each decode a part of its values and exchange them with v_permlane16/32_swap (1), or every
-// row decodes all 64 (0).
-#ifndef GL_FEAT_SPLIT
-#define GL_FEAT_SPLIT 1
-#endif
-// Cache policy of the remainder's X-row gathers (0 = default, 2 = non-temporal, 16 / 17 =
-// device / system scope). Measured on MI355X: non-temporal 13 % slower at 32 columns, 2x for
-// the one-column remainder and 59 % for the 64-column PPR step (X is served from the
-// Infinity Cache, which these loads then skip); 16 and 17 11 % slower at 2 and 8 columns and
-// 32 % for the 64-column PPR step. Default.
-#ifndef GL_REM_X_CP
-#define GL_REM_X_CP 0
-#endif
-// Non-temporal stores of finished output rows (1 = on; measured slower, off by default).
-#ifndef GL_NT_STORE
-#define GL_NT_STORE 0
-#endif
-
 
 namespace gl {
 
@@ -138,18 +76,17 @@ struct RhsView {
 constexpr uint32_t kOob = 0x80000000u;        // never inside a descriptor (< 2 GB)
 constexpr int64_t kBufLimit = 0x7fff0000ll;    // largest array (bytes) for Ld<true>
 
-template <bool BUF, int CP = 0>
+template <bool BUF>
 struct Ld;
 
-// CP: cache-policy bits of every load (0 = default, 2 = non-temporal).
-template <int CP>
-struct Ld<true, CP> {
+template <>
+struct Ld<true> {
   __amdgpu_buffer_rsrc_t r;
   __device__ __forceinline__ Ld(const void* p, int64_t bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
                                             0x00020000)) {}
   __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, ok ? (uint32_t)i * 4u : kOob, sb, CP);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, ok ? (uint32_t)i * 4u : kOob, sb, 0);
   }
   __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
     return __builtin_bit_cast(float, u32(i, ok, sb));
@@ -158,37 +95,32 @@ struct Ld<true, CP> {
   // into buffer_load_dwordx2/x4); the vector-result buffer builtins are not used.
   __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
     const uint32_t o = ok ? (uint32_t)i * 8u : kOob;
-    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, CP);
-    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, CP);
+    const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0);
+    const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0);
     return ((uint64_t)hi << 32) | lo;
   }
   __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
     const uint32_t o = ok ? (uint32_t)i * 4u : kOob;
     return make_float4(
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, CP)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, CP)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 8u, 0, CP)),
-        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 12u, 0, CP)));
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 4u, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 8u, 0, 0)),
+        __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, o + 12u, 0, 0)));
   }
 };
 
-template <int CP>
-struct Ld<false, CP> {
+template <>
+struct Ld<false> {
   const char* p;
   __device__ __forceinline__ Ld(const void* q, int64_t) : p((const char*)q) {}
-  template <class T>
-  __device__ __forceinline__ T get(const char* a) const {
-    if constexpr (CP != 0) return __builtin_nontemporal_load((const T*)a);
-    else return *(const T*)a;
-  }
   __device__ __forceinline__ uint32_t u32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return ok ? get<uint32_t>(p + i * 4 + sb) : 0u;
+    return ok ? *(const uint32_t*)(p + i * 4 + sb) : 0u;
   }
   __device__ __forceinline__ float f32(int64_t i, bool ok, uint32_t sb = 0) const {
-    return ok ? get<float>(p + i * 4 + sb) : 0.f;
+    return ok ? *(const float*)(p + i * 4 + sb) : 0.f;
   }
   __device__ __forceinline__ uint64_t u64(int64_t i, bool ok) const {
-    return ok ? get<uint64_t>(p + i * 8) : 0ull;
+    return ok ? *(const uint64_t*)(p + i * 8) : 0ull;
   }
   __device__ __forceinline__ float4 f32x4(int64_t i, bool ok) const {
     return ok ? *(const float4*)(p + i * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -198,9 +130,9 @@ struct Ld<false, CP> {
 // The arrays of one SpMM as predicated loaders.
 template <bool BUF>
 struct PbrLd {
-  Ld<BUF, GL_STREAM_CP> codes, coords, offsets, data, rptr, rcol, rval;
+  Ld<BUF> codes, coords, offsets, data, rptr, rcol, rval;
   Ld<BUF> x;
-  Ld<BUF, GL_REM_X_CP> xr;  // X rows gathered by remainder entries
+  Ld<BUF> xr;  // X rows gathered by remainder entries
   __device__ __forceinline__ PbrLd(const struct PbrView& m, const RhsView& d);
 };
 
@@ -224,7 +156,6 @@ template <>
 struct TileFma<8> {
   static __device__ __forceinline__ void run(float (&a)[8], const float (&v)[4],
                                              const float (&x)[8]) {
-#if GL_FUSED_DPP_FMA
     // s_nop 1: two wait states between the VALU write of v[] and its first DPP read.
     asm("s_nop 1\n" GL_ROW8_LO(0, 8) GL_ROW8_HI(1, 8) GL_ROW8_LO(2, 9) GL_ROW8_HI(3, 9)
             GL_ROW8_LO(4, 10) GL_ROW8_HI(5, 10) GL_ROW8_LO(6, 11) GL_ROW8_HI(7, 11)
@@ -232,13 +163,6 @@ struct TileFma<8> {
           "+v"(a[6]), "+v"(a[7])
         : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(x[0]), "v"(x[1]), "v"(x[2]),
           "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]));
-#else
-#define GL_BC(r, c) a[r] = __builtin_fmaf(row_bcast<((r) * 8 + (c)) % 16>(v[((r) * 8 + (c)) / 16]), x[c], a[r])
-#define GL_BROW(r) GL_BC(r, 0); GL_BC(r, 1); GL_BC(r, 2); GL_BC(r, 3); GL_BC(r, 4); GL_BC(r, 5); GL_BC(r, 6); GL_BC(r, 7)
-    GL_BROW(0); GL_BROW(1); GL_BROW(2); GL_BROW(3); GL_BROW(4); GL_BROW(5); GL_BROW(6); GL_BROW(7);
-#undef GL_BROW
-#undef GL_BC
-#endif
   }
 };
 
@@ -246,7 +170,6 @@ template <>
 struct TileFma<4> {
   static __device__ __forceinline__ void run(float (&a)[4], const float (&v)[1],
                                              const float (&x)[4]) {
-#if GL_FUSED_DPP_FMA
 #define GL_F4(a_, n, xx) GL_F(a_, 4, xx, n)
     asm("s_nop 1\n"
         GL_F4(0, 0, 5) GL_F4(0, 1, 6) GL_F4(0, 2, 7) GL_F4(0, 3, 8)
@@ -256,14 +179,6 @@ struct TileFma<4> {
         : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3])
         : "v"(v[0]), "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]));
 #undef GL_F4
-#else
-#define GL_BC(r, c) a[r] = __builtin_fmaf(row_bcast<(r) * 4 + (c)>(v[0]), x[c], a[r])
-    GL_BC(0, 0); GL_BC(0, 1); GL_BC(0, 2); GL_BC(0, 3);
-    GL_BC(1, 0); GL_BC(1, 1); GL_BC(1, 2); GL_BC(1, 3);
-    GL_BC(2, 0); GL_BC(2, 1); GL_BC(2, 2); GL_BC(2, 3);
-    GL_BC(3, 0); GL_BC(3, 1); GL_BC(3, 2); GL_BC(3, 3);
-#undef GL_BC
-#endif
   }
 };
 
@@ -271,16 +186,9 @@ template <>
 struct TileFma<2> {
   static __device__ __forceinline__ void run(float (&a)[2], const float (&v)[1],
                                              const float (&x)[2]) {
-#if GL_FUSED_DPP_FMA
     asm("s_nop 1\n" GL_F(0, 2, 3, 0) GL_F(0, 2, 4, 1) GL_F(1, 2, 3, 2) GL_F(1, 2, 4, 3)
         : "+v"(a[0]), "+v"(a[1])
         : "v"(v[0]), "v"(x[0]), "v"(x[1]));
-#else
-    a[0] = __builtin_fmaf(row_bcast<0>(v[0]), x[0], a[0]);
-    a[0] = __builtin_fmaf(row_bcast<1>(v[0]), x[1], a[0]);
-    a[1] = __builtin_fmaf(row_bcast<2>(v[0]), x[0], a[1]);
-    a[1] = __builtin_fmaf(row_bcast<3>(v[0]), x[1], a[1]);
-#endif
   }
 };
 #undef GL_ROW8_LO
@@ -316,7 +224,6 @@ struct EpiStore {
   __device__ __forceinline__ void put(int, int64_t row, int64_t col, float acc) {
     float* p = y + elem(row, ldy, col, u24);
     if (accumulate) *p += acc;
-    else if (GL_NT_STORE) __builtin_nontemporal_store(acc, p);
     else *p = acc;
   }
   __device__ __forceinline__ void finish(int) {}
@@ -405,8 +312,7 @@ struct EpiPpr {
       res += fabsf(xn - x0);
       if (dmask != nullptr && dmask[row]) dang += xn;
     }
-    if (GL_NT_STORE) __builtin_nontemporal_store(xn, xnext + ix);
-    else xnext[ix] = xn;
+    xnext[ix] = xn;
   }
   // slot: the lane's column minus k0 (or -1). The workgroup's partial of a column is the
   // sum of its lanes' values taken in thread order, so partials (and the residuals built
@@ -480,8 +386,8 @@ __device__ __forceinline__ PbrLd<BUF>::PbrLd(const PbrView& m, const RhsView& d)
 
 // N (1..8) consecutive floats of X starting at element i, zeros for lanes with valid <= 0.
 // `wide` (wave-uniform: 16-byte aligned rows and a full column chunk) selects float4 loads.
-template <int N, bool BUF, int CP>
-__device__ __forceinline__ void load_row(const Ld<BUF, CP>& xl, int64_t i, bool wide, int valid,
+template <int N, bool BUF>
+__device__ __forceinline__ void load_row(const Ld<BUF>& xl, int64_t i, bool wide, int valid,
                                          float (&out)[N]) {
   if constexpr (N >= 4) {
     if (wide) {
@@ -575,12 +481,8 @@ __device__ __forceinline__ RowSpan load_row_span(const PbrLd<BUF>& ld, const Pbr
 // launches always; plain-pointer launches for aligned rows and a full column chunk).
 template <int B, int KW, bool TILES, bool REM, bool BUF, class Epi, int UF, bool VEC>
 __global__ void __launch_bounds__(256)
-#if GL_FEAT_WAVES > 0
-__attribute__((amdgpu_waves_per_eu(GL_FEAT_WAVES)))
-#else
 // the 64-column kernel is held to 128 VGPRs (4 waves per SIMD)
 __attribute__((amdgpu_waves_per_eu(KW == 64 ? GL_FEAT64_WAVES : 1)))
-#endif
 k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int G = kWave / KW;           // tiles in flight per wave
   constexpr int P = B * B;                // bits per tile
@@ -593,12 +495,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   constexpr int EBW = KW >= 64 ? (BUF ? GL_REM_EB64 : GL_REM_EB64_PTR) : GL_REM_EB;
   constexpr int EB = L < EBW ? L : EBW;
   static_assert(L % EB == 0, "remainder batches must not straddle a block of L entries");
-  // flat remainder: LE lanes (4 columns each) per entry, NGR entry groups per wave
-  constexpr int LE = KW / 4;
-  constexpr int NGR = kWave / LE;
-  constexpr int EBF = GL_REM_FLAT_EB;
   __shared__ float racc[4][B * KW];
-  __shared__ int rsb_all[4][B + 1];
 
   if (!epi.active()) return;
   const PbrLd<BUF> ld(m, d);
@@ -610,7 +507,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   // Split decode: the R = 4 / G DPP rows of a tile each decode NL = NV / R of its 16-position
   // chunks (row ri takes chunks ri * NL ..), and permlane swaps give every row all of them.
   // (buffer-descriptor kernels only: with plain-pointer loads it measured 3 % slower at 16M rows)
-  constexpr bool kSplit = GL_FEAT_SPLIT && B == 8 && G <= 2 && BUF;
+  constexpr bool kSplit = B == 8 && G <= 2 && BUF;
   constexpr int R = kSplit ? 4 / G : 1;
   constexpr int NL = NV / R;
   const int ri = (lane >> 4) & (R - 1);
@@ -636,10 +533,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int nvq = kvalid - kq;
   const int gbase = lane & ~(L - 1);
   float* __restrict__ ra = racc[wid];
-  int* __restrict__ rsb = rsb_all[wid];
-  const int eg = lane / LE;             // flat remainder: entry group of this lane
-  const int kq4 = (lane % LE) * 4;      // flat remainder: first of this lane's 4 columns
-  const int nv4 = kvalid - kq4;
 
   // Values and X rows of one batch of G*U tiles (U groups of G tiles, one tile per group).
   struct FeatBatch {
@@ -728,18 +621,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   TileMeta mt{0u, 0u, 0, 0};
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
-#if GL_REM_FLAT
-  // lane k <= B: remainder start of row r0 + k (clamped to the matrix)
-  int rsv = 0;
-  if (REM && s < sc.end) rsv = (int)ld.rptr.u32(min<int64_t>(s * B + lane, m.rows), lane <= B);
-#else
   if constexpr (REM) rs = load_row_span(ld, m, s * B + rr, s < sc.end);
-#endif
-#if GL_FEAT_PIPE
-  FeatBatch Pb;
-  if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
-  else Pb.clear();
-#endif
 
   for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
@@ -758,27 +640,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       tc0 = m.stripe_ptr[snn];
       tc1 = m.stripe_ptr[snn + 1];
     }
-#if GL_REM_FLAT
-    int rsvn = 0;
-    if (REM && sn < sc.end)
-      rsvn = (int)ld.rptr.u32(min<int64_t>(sn * B + lane, m.rows), lane <= B);
-    // this lane group's chunk of the stripe's remainder entries, and its first batch
-    int cb = 0, ce = 0;
-    int fcol[EBF];
-    float fval[EBF];
-    if constexpr (REM) {
-      const int ea = __shfl(rsv, 0, kWave), ee = __shfl(rsv, B, kWave);
-      const int cs = (ee - ea + NGR - 1) / NGR;
-      cb = min(ee, ea + eg * cs);
-      ce = min(ee, cb + cs);
-#pragma unroll
-      for (int i = 0; i < EBF; ++i) {
-        const bool ok = cb + i < ce;
-        fcol[i] = (int)ld.rcol.u32(cb + i, ok);
-        fval[i] = ld.rval.f32(cb + i, ok);
-      }
-    }
-#else
     if constexpr (REM) rsn = load_row_span(ld, m, sn * B + rr, sn < sc.end);
     // ---- first remainder entries of this stripe
     int scol = 0;
@@ -788,34 +649,12 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       scol = (int)ld.rcol.u32(rs.j0 + qq, ok);
       sval = ld.rval.f32(rs.j0 + qq, ok);
     }
-#endif
 
     float acc[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) acc[r] = 0.f;
 
     if constexpr (TILES) {
-#if GL_FEAT_PIPE
-      // Pb holds the loads of this stripe's first batch, issued during the previous stripe.
-      for (int tc = ta0; tc < ta1; tc += kWave) {
-        const int tce = min(ta1, tc + kWave);
-        if (tc != ta0) {  // stripes of more than 64 tiles: next chunk starts unpipelined
-          mt = load_tile_meta(ld, tc, ta1, lane);
-          load_batch(Pb, mt, tc, tc, tce);
-        }
-        for (int tb = tc; tb < tce; tb += G * U) {
-          const int nb = tb + G * U;
-          FeatBatch Qb;
-          if (nb < tce) load_batch(Qb, mt, tc, nb, tce);
-          else if (tce == ta1 && tb0 < tb1) load_batch(Qb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
-          else Qb.clear();
-#pragma unroll
-          for (int u = 0; u < U; ++u) TileFma<B>::run(acc, Pb.v[u], Pb.x[u]);
-          Pb = Qb;
-        }
-      }
-      if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
-#else
       // Chunks of 64 tiles. The first uses the metadata prefetched a stripe ago; later ones
       // (stripes of more than 64 tiles) load their own, in a loop of their own, so the
       // first chunk never waits on a metadata load of this stripe.
@@ -831,70 +670,9 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
       if (ta0 < ta1) chunk(mt, ta0);
       for (int tc = ta0 + kWave; tc < ta1; tc += kWave)
         chunk(load_tile_meta(ld, tc, ta1, lane), tc);
-#endif
     }
 
     if constexpr (REM) {
-#if GL_REM_FLAT
-      // Entry groups walk their chunk in order, keep one running sum per lane (4 columns)
-      // and add it into the wave's LDS row accumulator when the row changes. Only this wave
-      // touches its accumulator, so the order of the adds, and the result, is fixed.
-#pragma unroll
-      for (int i = lane; i < B * KW; i += kWave) ra[i] = 0.f;
-      if (lane <= B) rsb[lane] = rsv;
-      wave_lds_sync();
-      int row = 0;
-#pragma unroll
-      for (int k = 1; k < B; ++k) row += rsb[k] <= cb ? 1 : 0;
-      int nxt = rsb[row + 1];
-      float rac[4] = {0.f, 0.f, 0.f, 0.f};
-      for (int e = cb; e < ce; e += EBF) {
-        float xv[EBF][4];
-#pragma unroll
-        for (int i = 0; i < EBF; ++i)
-          load_row<4>(ld.xr, xrow<BUF>(fcol[i], d.ldx) + kq4, vec, e + i < ce ? nv4 : 0, xv[i]);
-        float av[EBF];
-#pragma unroll
-        for (int i = 0; i < EBF; ++i) av[i] = fval[i];
-        // the next batch's columns and values travel while this batch's rows do
-#pragma unroll
-        for (int i = 0; i < EBF; ++i) {
-          const bool ok = e + EBF + i < ce;
-          fcol[i] = (int)ld.rcol.u32(e + EBF + i, ok);
-          fval[i] = ld.rval.f32(e + EBF + i, ok);
-        }
-#pragma unroll
-        for (int i = 0; i < EBF; ++i) {
-          const int ei = e + i;
-          if (ei < ce) {
-            if (ei >= nxt) {
-              if (kq4 < KW) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) atomicAdd(&ra[row * KW + kq4 + u], rac[u]);
-              }
-#pragma unroll
-              for (int u = 0; u < 4; ++u) rac[u] = 0.f;
-              do {
-                ++row;
-                nxt = rsb[row + 1];
-              } while (ei >= nxt);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) rac[u] = __builtin_fmaf(av[i], xv[i][u], rac[u]);
-          }
-        }
-      }
-      if (cb < ce && kq4 < KW) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) atomicAdd(&ra[row * KW + kq4 + u], rac[u]);
-      }
-      wave_lds_sync();
-      if (g == 0) {
-#pragma unroll
-        for (int r = 0; r < B; ++r) acc[r] += ra[r * KW + f];
-      }
-      wave_lds_sync();
-#else
       // row lanes: the L lanes of row rr hold FPL columns each and walk that row's entries
       float racc2[FPL];
 #pragma unroll
@@ -934,7 +712,6 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
         for (int r = 0; r < B; ++r) acc[r] += ra[r * KW + f];
       }
       wave_lds_sync();
-#endif
     }
 
     // combine the tile groups: every group ends with the full row sums
@@ -960,11 +737,7 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     tb0 = tc0;
     tb1 = tc1;
     mt = mtn;
-#if GL_REM_FLAT
-    rsv = rsvn;
-#else
     rs = rsn;
-#endif
   }
   epi.finish(fok ? f : -1);
 }
@@ -981,8 +754,11 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   constexpr int G = kWave / P;      // tiles per wave
   constexpr int U = UB;
   constexpr int L = kWave / B;      // lanes per matrix row
-  // densely tiled 8x8 matrices (UB == 8, whole-stripe batches): row lanes (GL_BIT_ROWLANE)
-  constexpr bool ROWLANE = GL_BIT_ROWLANE && B == 8 && UB == 8 && KW == 1;
+  // Densely tiled 8x8 matrices (UB == 8, whole-stripe batches) at one RHS column: lane
+  // (row r, slot q) walks the 8 bits of row r of tile q of each 8-tile pass, instead of one
+  // lane per tile bit. Measured on MI355X (1M x 1M, 8 tiles per stripe): tiles 76 vs 93 us,
+  // the fused SpMM 113.5 vs 128.5 us. On sparse tilings most of the 8 slots would stay empty.
+  constexpr bool ROWLANE = B == 8 && UB == 8 && KW == 1;
 
   if (!epi.active()) return;
   const PbrLd<BUF> ld(m, d);
@@ -1044,11 +820,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
   RowSpan rs{0, 0};
   if constexpr (REM) rs = load_row_span<KW == 1>(ld, m, s * B + rl, s < sc.end);
-#if GL_BIT_PIPE
-  BitBatch Pb;
-  if (TILES && ta0 < ta1) load_batch(Pb, mt, ta0, ta0, min(ta1, ta0 + kWave));
-  else Pb.clear();
-#endif
 
   for (; s < sc.end; s += sc.stride) {
     const int64_t r0 = s * B;
@@ -1119,29 +890,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
         }
       }
     } else if constexpr (TILES) {
-#if GL_BIT_PIPE
-      // Pb holds the loads of this stripe's first batch, issued during the previous stripe.
-      for (int tc = ta0; tc < ta1; tc += kWave) {
-        const int tce = min(ta1, tc + kWave);
-        if (tc != ta0) {  // stripes of more than 64 tiles: next chunk starts unpipelined
-          mt = load_tile_meta(ld, tc, ta1, lane);
-          load_batch(Pb, mt, tc, tc, tce);
-        }
-        for (int tb = tc; tb < tce; tb += G * U) {
-          const int nb = tb + G * U;
-          BitBatch Qb;
-          if (nb < tce) load_batch(Qb, mt, tc, nb, tce);
-          else if (tce == ta1 && tb0 < tb1) load_batch(Qb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
-          else Qb.clear();
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int k = 0; k < KW; ++k) acc[k] = __builtin_fmaf(Pb.v[u], Pb.x[u][k], acc[k]);
-          Pb = Qb;
-        }
-      }
-      if (ta0 == ta1 && tb0 < tb1) load_batch(Pb, mtn, tb0, tb0, min(tb1, tb0 + kWave));
-#else
       // chunks of 64 tiles, the first with the prefetched metadata (as in the feature kernel)
       auto chunk = [&](const TileMeta& meta, int tc) {
         const int tce = min(ta1, tc + kWave);
@@ -1157,7 +905,6 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
       if (ta0 < ta1) chunk(mt, ta0);
       for (int tc = ta0 + kWave; tc < ta1; tc += kWave)
         chunk(load_tile_meta(ld, tc, ta1, lane), tc);
-#endif
     }
 
     if constexpr (REM) {

@@ -15,6 +15,7 @@ pairs within two hops are evaluated one by one.
 from __future__ import annotations
 
 import ctypes
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -177,6 +178,16 @@ def predict(params, wi: WsInputs, near_cap=None):
     return x0, n_near
 
 
+def run_warm(params, pbr, seed_sets, *, graph=None, alpha=0.85, **run_kw):
+    """PPR from the learned start: features, network, then run_ppr from X0 (run_kw as for
+    run_ppr, alpha included). Returns run_ppr's result."""
+    from .ppr import run_ppr
+    wi = inputs(pbr, seed_sets, alpha, graph=graph)
+    x0, _ = predict(params, wi)
+    del wi
+    return run_ppr(pbr, seed_sets, alpha=alpha, x0=x0, **run_kw)
+
+
 def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0,
                    near_samples=None):
     """Training set of one graph: the pairs within two hops (all of them, or
@@ -270,6 +281,11 @@ def grad(params, feat, target, weight):
                                 g.data_ptr(), loss.data_ptr(), _native.stream_handle(dev)),
                   "gl_mlp_grad")
     return g, float(loss.item())
+
+
+# the committed weights, trained on the clustered generator's 1M-node / 16M-edge graphs
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                       "warmstart_clustered.npz")
 
 
 def save(params, path):
