@@ -34,7 +34,7 @@ def test_config1_sbm_against_fp64(device):
     p = np.zeros(n)
     p[seed] = 1.0
     exact = oppr.solve(A64, p, 0.85, dang, "personalization")
-    x_nx = oppr.nx_pagerank(A_col, p, 0.85, tol=1e-12, max_iter=1000)
+    x_nx = oppr.nx_pagerank(A_col, p, 0.85, tol=1e-16, max_iter=1000)  # stop at N*tol
     assert np.abs(x_nx - exact).sum() < 1e-9  # the oracle pair agrees first
     pbr = graph_link.csr_to_pbr(A32, 8, 8, 4).to(device)
     for tol in (1.5e-7, 1e-8):
