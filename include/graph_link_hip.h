@@ -136,6 +136,10 @@ typedef struct gl_ppr_opts {
   int32_t kw;          /* RHS columns per launch, 0 = library choice */
   int32_t grid_mult;   /* workgroups per CU of the persistent launch, 0 = as many as the
                           device holds of the step kernel at once */
+  int32_t extrapolate_every; /* 0 = plain power iteration (the reference); P >= 2 = after
+                          passes P, 2P, ... each active column whose residual ratio rho =
+                          r_t / r_{t-1} lies in [0.5, 0.97] is extrapolated,
+                          X += rho / (1 - rho) (X_t - X_{t-1}) (at most 4x), on the device */
 } gl_ppr_opts;
 
 int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,

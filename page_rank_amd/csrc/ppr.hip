@@ -97,6 +97,8 @@ struct ColState {
   float* dsum;
   float* cu;
   float* tele;
+  float* res_prev;  // residual of the previous pass (extrapolation)
+  float* beta;      // extrapolation factor of the pass just retired (0 = none)
 };
 
 __device__ __forceinline__ void set_coeffs(const ColState& c, int k, float alpha, int64_t N,
@@ -112,11 +114,14 @@ __global__ void k_ctl_init(PprCtl* ctl, ColState c, int64_t K, float alpha, int6
     c.frozen[k] = 0;
     c.iters[k] = 0;
     c.conv[k] = 0;
+    c.res_prev[k] = 0.f;
+    c.beta[k] = 0.f;
     set_coeffs(c, k, alpha, N, dmode);
   }
   if (threadIdx.x == 0) {
     ctl->iter = 0;
     ctl->active = (int32_t)K;
+    ctl->kick = 0;
   }
 }
 
@@ -132,17 +137,22 @@ constexpr int kUnroll = 8;
 __global__ void __launch_bounds__(kRetireThreads)
 k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ part_res,
          const float* __restrict__ part_dang, float* __restrict__ res, float* __restrict__ hist,
-         float tol, float alpha, int64_t N, int dmode, int together, int max_iter) {
+         float tol, float alpha, int64_t N, int dmode, int together, int max_iter,
+         int extrap_every) {
   __shared__ double s_r[kRetireThreads];
   __shared__ double s_d[kRetireThreads];
   __shared__ int s_max_bits;
   __shared__ int s_active;
+  __shared__ int s_kick;
   if (ctl->active <= 0) return;  // the pass was skipped: nothing to retire
   const int it = ctl->iter;
   if (threadIdx.x == 0) {
     s_max_bits = 0;
     s_active = 0;
+    s_kick = 0;
   }
+  // an extrapolation follows passes extrap_every, 2 * extrap_every, ...
+  const bool kick_pass = extrap_every > 0 && (it + 1) % extrap_every == 0;
   int KC = 64;  // columns per sweep: smallest power of two >= K, at most 64
   while (KC > 1 && KC / 2 >= K) KC /= 2;
   const int S = kRetireThreads / KC;  // slices
@@ -190,6 +200,9 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
       const float rf = (float)s_r[kk];
       res[k] = rf;  // read back below by other threads of this workgroup
       if (!c.frozen[k]) {
+        // beta[k] keeps the previous iterate's dangling mass until the sweep below sets
+        // the column's extrapolation factor
+        c.beta[k] = c.dsum[k];
         c.dsum[k] = (float)s_d[kk];
         atomicMax(&s_max_bits, __float_as_int(rf));  // rf >= 0: int order == float order
       }
@@ -206,19 +219,57 @@ k_retire(PprCtl* ctl, ColState c, int64_t K, int nwg, const float* __restrict__ 
     const float r = res[k];
     if (hist) hist[(int64_t)it * K + k] = r;
     const bool conv = together ? (maxres < tol) : (r < tol);
+    const float dsum_prev = c.beta[k];
+    float beta = 0.f;
     if (conv || last) {
       c.frozen[k] = 1;
       c.iters[k] = it + 1;
       c.conv[k] = conv ? 1 : 0;
     } else {
       atomicAdd(&s_active, 1);
+      // Extrapolation of the dominant error mode (Aitken's delta-squared on the residual
+      // ratio, as Kamvar et al. apply it to PageRank): once the residual shrinks by a
+      // steady ratio rho per pass, the remaining error is about rho / (1 - rho) times the
+      // last update, so X_{t+1} += beta (X_{t+1} - X_t) with beta = rho / (1 - rho). Only
+      // for slowly mixing columns (rho in [0.5, 0.97]), capped at 4.
+      const float rp = c.res_prev[k];
+      if (kick_pass && rp > 0.f) {
+        const float rho = r / rp;
+        if (rho >= 0.5f && rho <= 0.97f) beta = fminf(rho / (1.f - rho), 4.f);
+      }
+      if (beta > 0.f) {
+        // the dangling mass is linear in X: that of the extrapolated iterate
+        c.dsum[k] = c.dsum[k] + beta * (c.dsum[k] - dsum_prev);
+        s_kick = 1;  // benign race: every writer stores 1
+      }
       set_coeffs(c, (int)k, alpha, N, dmode);
     }
+    c.beta[k] = beta;
+    c.res_prev[k] = r;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     ctl->iter = it + 1;
     ctl->active = s_active;
+    ctl->kick = s_kick;
+  }
+}
+
+// The extrapolation step that k_retire scheduled: xn[i][k] += beta[k] (xn[i][k] - xc[i][k])
+// for every column with beta > 0 (xn = X_{t+1}, the next pass's input; xc = X_t). A grid
+// stride loop over rows, consecutive threads on consecutive columns; exits at once when no
+// column extrapolates.
+__global__ void __launch_bounds__(256) k_extrapolate(const PprCtl* __restrict__ ctl,
+                                                     const float* __restrict__ beta,
+                                                     const float* __restrict__ xc,
+                                                     float* __restrict__ xn, int64_t N,
+                                                     int64_t K) {
+  if (!ctl->kick) return;
+  const int64_t total = N * K;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float b = beta[e % K];
+    if (b > 0.f) xn[e] = __builtin_fmaf(b, xn[e] - xc[e], xn[e]);
   }
 }
 
@@ -232,6 +283,8 @@ struct Workspace {
   float* cu;
   float* tele;
   float* res;
+  float* res_prev;
+  float* beta;
   float* part_res;
   float* part_dang;
   int32_t* sptr;
@@ -257,6 +310,8 @@ static Workspace carve(char* base, int64_t rows, int32_t B, int64_t K, int64_t E
   w.cu = (float*)take(K * 4);
   w.tele = (float*)take(K * 4);
   w.res = (float*)take(K * 4);
+  w.res_prev = (float*)take(K * 4);
+  w.beta = (float*)take(K * 4);
   w.part_res = (float*)take((int64_t)kMaxWg * K * 4);
   w.part_dang = (float*)take((int64_t)kMaxWg * K * 4);
   w.sptr = (int32_t*)take((ns + 1) * 4);
@@ -309,6 +364,8 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   GL_REQUIRE(stripe_ptr && rem_indptr, "null stripe_ptr / remainder indptr");
   GL_REQUIRE(num_tiles == 0 || (codes && coords && offsets && data), "null tile arrays");
   GL_REQUIRE(opts->max_iter >= 0, "max_iter must be >= 0");
+  GL_REQUIRE(opts->extrapolate_every >= 0 && opts->extrapolate_every != 1,
+             "extrapolate_every must be 0 (off) or >= 2");
   GL_REQUIRE(opts->dangling_mode >= 0 && opts->dangling_mode <= 2, "bad dangling_mode");
   GL_REQUIRE(opts->dangling_mode == GL_DANGLING_NONE || dangling_mask != nullptr,
              "dangling_mask required for dangling_mode %d", opts->dangling_mode);
@@ -397,7 +454,7 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
                        (const float*)w.part_res, nb, K, w.dsum);
     GL_LAUNCH_CHECK("k_dangling_final");
   }
-  ColState cs{w.frozen, iterations, converged, w.dsum, w.cu, w.tele};
+  ColState cs{w.frozen, iterations, converged, w.dsum, w.cu, w.tele, w.res_prev, w.beta};
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(1024), 0, st, w.ctl, cs, K, opts->alpha, rows,
                      opts->dangling_mode);
   GL_LAUNCH_CHECK("k_ctl_init");
@@ -431,6 +488,9 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   const int together = opts->retire_mode == GL_RETIRE_TOGETHER ? 1 : 0;
 
   float* bufs[2] = {X_out, w.xb};
+  const int extrap_blocks = blocks_for(NK, 256) < 8 * device_info().num_cus
+                                ? blocks_for(NK, 256)
+                                : 8 * device_info().num_cus;
   auto one_iteration = [&](int parity) -> int {
     const float* xc = bufs[parity];
     float* xn = bufs[parity ^ 1];
@@ -446,8 +506,13 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
     }
     hipLaunchKernelGGL(k_retire, dim3(1), dim3(kRetireThreads), 0, st, w.ctl, cs, K, nwg,
                        w.part_res, w.part_dang, w.res, history, opts->tol, opts->alpha, rows,
-                       opts->dangling_mode, together, opts->max_iter);
+                       opts->dangling_mode, together, opts->max_iter, opts->extrapolate_every);
     GL_LAUNCH_CHECK("k_retire");
+    if (opts->extrapolate_every > 0) {
+      hipLaunchKernelGGL(k_extrapolate, dim3(extrap_blocks), dim3(256), 0, st, w.ctl, w.beta,
+                         xc, xn, rows, K);
+      GL_LAUNCH_CHECK("k_extrapolate");
+    }
     return GL_OK;
   };
 

@@ -233,7 +233,8 @@ struct EpiStore {
 struct PprCtl {
   int32_t iter;    // completed iterations
   int32_t active;  // columns still iterating
-  int32_t pad[2];
+  int32_t kick;    // the pass just retired is followed by an extrapolation
+  int32_t pad;
 };
 
 // Fused power-iteration step:
