@@ -293,7 +293,20 @@ def ppr_leg(args, dev):
                        "iterations_max": res6.total_iterations,
                        "iterations_mean": round(float(res6.iterations.mean()), 2)}
     log(f"  plain tol 1e-6: {K / t6:.1f} PPR/s, iterations max {res6.total_iterations}")
-    wres = None
+    # dominant-mode extrapolation every EXTRAP passes (run_ppr extrapolate_every)
+    EXTRAP = 6
+    te, eres = _timed_runs(lambda: gl.run_ppr(pbr, sets, tol=tol_acc, extrapolate_every=EXTRAP,
+                                              **kw))
+    out["extrapolated"] = {"tol": tol_acc, "extrapolate_every": EXTRAP,
+                           "ppr_per_s": round(K / te, 3), "ms": round(te * 1e3, 3),
+                           "iterations_max": eres.total_iterations,
+                           "iterations_mean": round(float(eres.iterations.mean()), 2),
+                           "converged": int(eres.converged.sum()),
+                           "speedup_vs_plain": round(t / te, 3)}
+    log(f"  extrapolated (every {EXTRAP}) tol {tol_acc}: {K / te:.1f} PPR/s, {te*1e3:.1f} ms, "
+        f"iterations max {eres.total_iterations} mean {eres.iterations.mean():.1f} "
+        f"({t / te:.2f}x plain)")
+    wres = wres_x = None
     wpath = os.path.join(ROOT, "page_rank_amd", "data", "warmstart_clustered.npz")
     if os.path.exists(wpath):
         from page_rank_amd import warmstart as ws
@@ -303,14 +316,29 @@ def ppr_leg(args, dev):
                                                    tol=tol_acc, **kw))
         out["learned"] = {"tol": tol_acc, "ppr_per_s": round(K / tw, 3),
                           "ms": round(tw * 1e3, 3),
-                          "iterations_max": wres.total_iterations,
-                          "iterations_mean": round(float(wres.iterations.mean()), 2),
+                          "iterations_max": wres.total_iterations + wres.feature_passes,
+                          "iterations_mean": round(float(wres.iterations.mean())
+                                                   + wres.feature_passes, 2),
+                          "feature_passes": wres.feature_passes, "blend": ws.BLEND,
                           "converged": int(wres.converged.sum()),
                           "speedup_vs_plain": round(t / tw, 3),
                           "model": "page_rank_amd/data/warmstart_clustered.npz"}
         log(f"  learned warm start tol {tol_acc}: {K / tw:.1f} PPR/s, {tw*1e3:.1f} ms, "
-            f"iterations max {wres.total_iterations} mean {wres.iterations.mean():.1f} "
-            f"({t / tw:.2f}x plain)")
+            f"iterations max {wres.total_iterations}+{wres.feature_passes} mean "
+            f"{wres.iterations.mean():.1f}+{wres.feature_passes} ({t / tw:.2f}x plain)")
+        twx, wres_x = _timed_runs(lambda: ws.run_warm(params, pbr, sets, graph=graph,
+                                                      tol=tol_acc, extrapolate_every=EXTRAP,
+                                                      **kw))
+        out["learned_extrapolated"] = {
+            "tol": tol_acc, "extrapolate_every": EXTRAP, "ppr_per_s": round(K / twx, 3),
+            "ms": round(twx * 1e3, 3),
+            "iterations_max": wres_x.total_iterations + wres_x.feature_passes,
+            "iterations_mean": round(float(wres_x.iterations.mean()) + wres_x.feature_passes, 2),
+            "converged": int(wres_x.converged.sum()),
+            "speedup_vs_plain": round(t / twx, 3),
+            "speedup_vs_extrapolated": round(te / twx, 3)}
+        log(f"  learned + extrapolated: {K / twx:.1f} PPR/s, {twx*1e3:.1f} ms "
+            f"({t / twx:.2f}x plain, {te / twx:.2f}x extrapolated)")
     if args.ppr_check_cols and not args.no_cpu:
         from oracle import ppr as oppr
         cols = np.random.default_rng(5).choice(K, min(K, args.ppr_check_cols), replace=False)
@@ -342,10 +370,12 @@ def ppr_leg(args, dev):
         out["l1_vs_fp64"], out["max_abs_vs_fp64"] = errs(res)
         out["all_checked_below_1e-6"] = bool(max(out["l1_vs_fp64"]) < 1e-6)
         out["tol_1e-6"]["l1_vs_fp64"], out["tol_1e-6"]["max_abs_vs_fp64"] = errs(res6)
-        if wres is not None:
-            out["learned"]["l1_vs_fp64"], out["learned"]["max_abs_vs_fp64"] = errs(wres)
-            out["learned"]["all_checked_below_1e-6"] = bool(
-                max(out["learned"]["l1_vs_fp64"]) < 1e-6)
+        for key, r in (("extrapolated", eres), ("learned", wres),
+                       ("learned_extrapolated", wres_x)):
+            if r is None:
+                continue
+            out[key]["l1_vs_fp64"], out[key]["max_abs_vs_fp64"] = errs(r)
+            out[key]["all_checked_below_1e-6"] = bool(max(out[key]["l1_vs_fp64"]) < 1e-6)
         out["checked_columns"] = cols.tolist()
         out["cpu_baseline"] = {"value": round(len(cols) / t_cpu, 4), "unit": "PPR/s", "cores": 1,
                                "kind": "port",

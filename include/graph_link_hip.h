@@ -248,6 +248,12 @@ int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
                    uint64_t* near_count, void* stream);
 /* x0[pairs[m]] = val[m]. */
 int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, void* stream);
+/* x0 <- max(0, x4 + lam (x0 - x4)), elementwise over [N][K], where x4 is the 4th
+ * power-iteration iterate from the seed start rebuilt from the feature products
+ * (v5 = A^j P, j = 0..4): the four products the features needed count as the first four
+ * passes, and the network's start moves that iterate a fraction lam of the way. */
+int gl_ws_blend_iterate(float* x0, const float* const* v5, int64_t N, int64_t K, float alpha,
+                        float lam, void* stream);
 /* Scale every column of x [N][K] (K <= 1024) to sum 1. sums: K * 1025 floats of scratch; the
  * first K receive the column sums, taken in a fixed order (repeated calls agree bit for bit). */
 int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream);

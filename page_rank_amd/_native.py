@@ -97,6 +97,7 @@ _PROTOS = {
                                       c_i64, c_p, c_p]),
     "gl_ws_scatter": (ctypes.c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "gl_ws_normalize": (ctypes.c_int, [c_p, c_i64, c_i64, c_p, c_p]),
+    "gl_ws_blend_iterate": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_f32, c_f32, c_p]),
     "gl_csr_normalize_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "gl_csr_normalize_columns_f32": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
                                                     c_i64, c_p]),

@@ -415,6 +415,21 @@ __global__ void __launch_bounds__(1024) k_colsum_final(const float* __restrict__
     sums[k] = t;
   }
 }
+// x0 <- max(0, x4 + lam (x0 - x4)) with x4 = the 4th power-iteration iterate from the seed
+// start, rebuilt from the feature products v_j = A^j P (j = 0..4):
+// x4 = (1 - alpha) (v0 + alpha v1 + alpha^2 v2 + alpha^3 v3) + alpha^4 v4.
+__global__ void __launch_bounds__(256) k_blend_iterate(float* __restrict__ x0, const float* v0,
+                                                       const float* v1, const float* v2,
+                                                       const float* v3, const float* v4,
+                                                       int64_t n, float alpha, float lam) {
+  const float c = 1.f - alpha, a2 = alpha * alpha, a3 = a2 * alpha, a4 = a3 * alpha;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float x4 = c * (v0[e] + alpha * v1[e] + a2 * v2[e] + a3 * v3[e]) + a4 * v4[e];
+    x0[e] = fmaxf(0.f, x4 + lam * (x0[e] - x4));
+  }
+}
+
 __global__ void k_colscale(float* __restrict__ x, int64_t N, int64_t K, const float* __restrict__ sums) {
   const int lane = threadIdx.x & 63;
   const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -570,6 +585,16 @@ int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, 
   hipLaunchKernelGGL(k_scatter_pairs, dim3((unsigned)((M + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, pairs, val, M, x0);
   GL_LAUNCH_CHECK("k_scatter_pairs");
+  return GL_OK;
+}
+
+int gl_ws_blend_iterate(float* x0, const float* const* v, int64_t N, int64_t K, float alpha,
+                        float lam, void* stream) {
+  GL_REQUIRE(x0 && v && N >= 1 && K >= 1, "bad arguments");
+  for (int j = 0; j < 5; ++j) GL_REQUIRE(v[j] != nullptr, "feature product %d is null", j);
+  hipLaunchKernelGGL(k_blend_iterate, dim3(nblocks(N * K, 256, 16384)), dim3(256), 0,
+                     (hipStream_t)stream, x0, v[0], v[1], v[2], v[3], v[4], N * K, alpha, lam);
+  GL_LAUNCH_CHECK("k_blend_iterate");
   return GL_OK;
 }
 

@@ -46,6 +46,7 @@ class PPRResult:
     converged: np.ndarray        # [K] bool
     history: np.ndarray          # [total_iterations, K] L1 residual (0 once a column stopped)
     total_iterations: int
+    feature_passes: int = 0      # passes done before the run (a warm start's feature products)
 
 
 def _seed_arrays(seed_sets, weights=None):

@@ -145,8 +145,17 @@ def pair_features(wi: WsInputs, pairs):
     return feat
 
 
-def predict(params, wi: WsInputs, near_cap=None):
-    """Normalized warm-start matrix X0 [N][K] and the number of per-pair evaluations."""
+# Weight of the network's start against the 4th power iterate that its feature products
+# already hold (gl_ws_blend_iterate). Measured on the 4M-node clustered graph (64 seed sets,
+# tol 1.5e-7): lam 0 (the iterate alone) 41 passes after the 4 products, 0.25 37, 0.5 42,
+# 0.75 44, 1 (the network alone, no products counted) 45.
+BLEND = 0.25
+
+
+def predict(params, wi: WsInputs, near_cap=None, blend=BLEND):
+    """Normalized warm-start matrix X0 [N][K] and the number of per-pair evaluations.
+    blend: None = the network's start alone; lam = the 4th power iterate moved lam of the way
+    toward it (the feature products then count as the run's first four passes)."""
     L = _lib()
     dev = wi.v[0].device
     st = _native.stream_handle(dev)
@@ -165,7 +174,7 @@ def predict(params, wi: WsInputs, near_cap=None):
                   "gl_ws_assemble")
     n_near = int(cnt.item())
     if n_near > cap:
-        return predict(params, wi, near_cap=n_near)
+        return predict(params, wi, near_cap=n_near, blend=blend)
     pairs = near[:n_near]
     if n_near:
         vals = forward(params, pair_features(wi, pairs))
@@ -175,17 +184,25 @@ def predict(params, wi: WsInputs, near_cap=None):
     sums = torch.empty(wi.K * 1025, dtype=torch.float32, device=dev)
     _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
                   "gl_ws_normalize")
+    if blend is not None:
+        _native.check(L.gl_ws_blend_iterate(x0.data_ptr(), wi.vptrs(), wi.N, wi.K, wi.alpha,
+                                            float(blend), st), "gl_ws_blend_iterate")
+        _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
+                      "gl_ws_normalize")
     return x0, n_near
 
 
-def run_warm(params, pbr, seed_sets, *, graph=None, alpha=0.85, **run_kw):
+def run_warm(params, pbr, seed_sets, *, graph=None, alpha=0.85, blend=BLEND, **run_kw):
     """PPR from the learned start: features, network, then run_ppr from X0 (run_kw as for
-    run_ppr, alpha included). Returns run_ppr's result."""
+    run_ppr). Returns run_ppr's result; with a blend, the four feature products are passes of
+    the same power iteration, and `feature_passes` on the result says so (4, else 0)."""
     from .ppr import run_ppr
     wi = inputs(pbr, seed_sets, alpha, graph=graph)
-    x0, _ = predict(params, wi)
+    x0, _ = predict(params, wi, blend=blend)
     del wi
-    return run_ppr(pbr, seed_sets, alpha=alpha, x0=x0, **run_kw)
+    res = run_ppr(pbr, seed_sets, alpha=alpha, x0=x0, **run_kw)
+    res.feature_passes = 4 if blend is not None else 0
+    return res
 
 
 def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0,

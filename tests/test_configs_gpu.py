@@ -75,7 +75,8 @@ def test_config4_heldout_graph_warm_and_plain(device):
     for name, r in (("plain", plain), ("warm", warm)):
         d = np.abs(r.X[:, sel].cpu().numpy().astype(np.float64) - ref)
         l1, mx = d.sum(0), d.max(0)
-        print(f"{name}: iterations max {r.total_iterations} mean {r.iterations.mean():.1f}, "
+        print(f"{name}: iterations max {r.total_iterations}+{r.feature_passes} mean "
+              f"{r.iterations.mean():.1f}+{r.feature_passes}, "
               f"L1 vs FP64 max {l1.max():.2e}, max abs {mx.max():.2e}")
         assert l1.max() < 1e-6
     print(f"wall: plain {(t2 - t1) * 1e3:.1f} ms, warm {(t3 - t2) * 1e3:.1f} ms "
