@@ -145,7 +145,7 @@ def test_assembly_lists_every_near_pair_once(device):
     p = ws.init_params(3, device)
     wi = ws.inputs(pbr, sets, 0.85)
     want = int(((wi.v[0] > 0) | (wi.v[1] > 0) | (wi.v[2] > 0)).sum().item())
-    x0, n_near = ws.predict(p, wi)
+    x0, n_near = ws.predict(p, wi, blend=None)
     assert n_near == want and n_near > n * K // 4
     assert bool((x0 > 0).all())
     # a sample of pairs through the per-pair path: ratio to X0 is 1 / column sum
@@ -159,7 +159,16 @@ def test_assembly_lists_every_near_pair_once(device):
         assert np.allclose(r, r[0], rtol=2e-5), k
     assert torch.allclose(x0.sum(0), torch.ones(K, device=device), atol=1e-4)
     # a near list smaller than the pair count is refused and predict retries with the count
-    x1, n1 = ws.predict(p, wi, near_cap=1 << 16)
+    x1, n1 = ws.predict(p, wi, near_cap=1 << 16, blend=None)
     assert n1 == want
     # every column sum is taken in a fixed order: the start vector repeats bit for bit
     assert torch.equal(x1, x0)
+    # the default start: the 4th power iterate moved BLEND of the way toward x0, renormalized
+    xb, _ = ws.predict(p, wi)
+    v = [t.double() for t in wi.v]
+    a = 0.85
+    x4 = (1 - a) * (v[0] + a * v[1] + a * a * v[2] + a ** 3 * v[3]) + a ** 4 * v[4]
+    want_b = (x4 + ws.BLEND * (x0.double() - x4)).clamp_min(0)
+    want_b = want_b / want_b.sum(0, keepdim=True)
+    assert torch.allclose(xb.double(), want_b, rtol=1e-4, atol=1e-9)
+    assert torch.allclose(xb.sum(0), torch.ones(K, device=device), atol=1e-4)
