@@ -52,6 +52,11 @@ def parse():
     ap.add_argument("--ppr-sets", type=int, default=64)
     ap.add_argument("--ppr-set-max", type=int, default=100,
                     help="seed-set sizes are uniform in 1..this")
+    ap.add_argument("--ppr-intra", type=float, default=0.8,
+                    help="fraction of the PPR graph's edges inside their community")
+    ap.add_argument("--ppr-profile-passes", type=int, default=0,
+                    help="profiling aid: build the PPR graph, run this many plain passes of "
+                         "the step (eager launches, no stop) twice, print their time, exit")
     ap.add_argument("--ppr-check-cols", type=int, default=8,
                     help="columns checked against the FP64 CPU power iteration (and timed "
                          "as the CPU baseline of the PPR leg)")
@@ -248,7 +253,8 @@ def ppr_leg(args, dev):
     from page_rank_amd import graphs
     n, m, K = args.ppr_nodes, args.ppr_edges, args.ppr_sets
     log(f"generating the {n} node / {m} edge clustered graph ...")
-    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000), seed=3)
+    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000),
+                                      intra=args.ppr_intra, seed=3)
     log("  edges drawn; building the column-stochastic CSR ...")
     A, dang = graphs.transition_matrix(src, dst, n)
     del src, dst
@@ -264,6 +270,15 @@ def ppr_leg(args, dev):
     smax = args.ppr_set_max
     sets = graphs.seed_sets(n, K, 1, smax, seed=4)
     dmode = "uniform" if dang.any() else "none"
+    if args.ppr_profile_passes:
+        np_ = args.ppr_profile_passes
+        pk = dict(tol=0.0, max_iter=np_, kw=64 if K >= 64 else 0, graph_iters=0,
+                  return_history=False, dangling=dang if dang.any() else None,
+                  dangling_mode=dmode)
+        tp, _ = _timed_runs(lambda: gl.run_ppr(pbr, sets, **pk), reps=1)
+        log(f"  {np_} plain passes: {tp / np_ * 1e3:.3f} ms per pass")
+        return {"profile_passes": np_, "ms_per_pass": round(tp / np_ * 1e3, 4),
+                "intra": args.ppr_intra, "nodes": n, "edges_nnz": int(A.nnz)}
     tuned = {k: v for k, v in gl.autotune.tune_ppr(pbr, K).items() if k != "table"}
     log(f"  autotuned launch: {tuned}")
     kw = dict(alpha=0.85, max_iter=300, dangling=dang if dang.any() else None,
