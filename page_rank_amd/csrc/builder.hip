@@ -205,6 +205,10 @@ constexpr int kHashWavesPerSimd = 7;
 // Grid of the count pass, in workgroups per CU: 7 of the 8 its LDS allows (0.71 ms for the
 // whole 1M x 1M build, against 0.77 ms with 8 and with 4).
 constexpr int kHashCountWgPerCu = 7;
+// Chunks of 64 entries per stripe whose column (and, in the fill pass, value) loads leave
+// together at the top of the stripe and stay in registers for both sweeps: one memory round
+// trip instead of one per chunk per sweep. Longer stripes load the rest chunk by chunk.
+constexpr int kHashPre = 4;
 
 // FILL: the fill pass's arrays (value offsets, slots in key order) exist only there, so the
 // count pass needs less LDS per wave and holds more waves. A block's first row is not kept:
@@ -255,10 +259,36 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
   const int br = a.br, bc = a.bc;
   const uint64_t below_lane = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
+  // Per-stripe loads that do not depend on the stripe's entries: CSR row starts (row k of the
+  // stripe in lane k, row 64 in lane 0's second register) and, in the fill pass, the stripe's
+  // output offsets and its rows' remainder offsets. They are issued a stripe ahead.
+  struct StripeHead {
+    int32_t rp0, rp1, nr;
+    int32_t nt, nv, nd;
+  };
+  auto load_head = [&](int64_t st) {
+    StripeHead hd{0, 0, 0, 0, 0, 0};
+    if (st >= a.num_stripes) return hd;
+    const int64_t rb = st * br;
+    if (lane <= br) hd.rp0 = a.indptr[min<int64_t>(rb + lane, a.rows)];
+    if (lane == 0 && br >= kWave) hd.rp1 = a.indptr[min<int64_t>(rb + kWave, a.rows)];
+    if constexpr (FILL) {
+      if (lane < br && rb + lane < a.rows) hd.nr = o.nr[rb + lane];
+      hd.nt = o.nt[st];
+      hd.nv = o.nv[st];
+      hd.nd = o.nd[st];
+    }
+    return hd;
+  };
+  StripeHead head = load_head(wave);
+
   for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
     const int64_t r0 = s * br;
-    for (int k = lane; k <= br; k += kWave) h.rows[k] = a.indptr[min<int64_t>(r0 + k, a.rows)];
-    if (lane < br) h.rcnt[lane] = 0;
+    const StripeHead cur = head;
+    if (lane <= br) h.rows[lane] = cur.rp0;
+    if (lane == 0 && br >= kWave) h.rows[kWave] = cur.rp1;
+    // fill: remainder entries of each row so far start at the row's remainder offset
+    if (lane < br) h.rcnt[lane] = FILL ? cur.nr : 0;
     for (int i = lane; i < kHashCap; i += kWave) {
       h.key[i] = -1;
       h.lo[i] = 0u;
@@ -266,7 +296,18 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     }
     if (lane == 0) h.flag = 0;
     wsync();
-    const int32_t E0 = h.rows[0], E1 = h.rows[br];
+    const int32_t E0 = __builtin_amdgcn_readfirstlane(h.rows[0]);
+    const int32_t E1 = __builtin_amdgcn_readfirstlane(h.rows[br]);
+    // this stripe's first kHashPre chunks, then the next stripe's head
+    int32_t pcol[kHashPre];
+    V pval[kHashPre];
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t e = E0 + k * kWave + lane;
+      pcol[k] = e < E1 ? a.indices[e] : 0;
+      if constexpr (FILL) pval[k] = e < E1 ? vals[e] : V(0);
+    }
+    head = load_head(s + nwaves);
     // row starts 1..7 in registers for the common block_rows <= 8 (a row is then found by
     // compares instead of a search in LDS)
     int32_t rs8[7];
@@ -283,31 +324,36 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     };
 
     // ---- insert every entry's block
-    for (int32_t c = E0; c < E1; c += kWave) {
-      const int32_t e = c + lane;
-      if (e < E1) {
-        const int32_t col = a.indices[e];
-        const int r = row_at(e);
-        const int32_t b = col / bc;
-        const int bit = r * bc + (col - b * bc);
-        int slot = hash_slot(b);
-        int probes = 0;
-        for (;;) {
-          const int32_t old = atomicCAS(&h.key[slot], -1, b);
-          if (old == -1 || old == b) break;
-          slot = (slot + 1) & (kHashCap - 1);
-          if (++probes == kHashCap) {
-            slot = -1;
-            break;
-          }
-        }
-        if (slot < 0) {
-          h.flag = 1;
-        } else {
-          if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
-          else atomicOr(&h.hi[slot], 1u << (bit - 32));
+    auto insert = [&](int32_t e, int32_t col) {
+      const int r = row_at(e);
+      const int32_t b = col / bc;
+      const int bit = r * bc + (col - b * bc);
+      int slot = hash_slot(b);
+      int probes = 0;
+      for (;;) {
+        const int32_t old = atomicCAS(&h.key[slot], -1, b);
+        if (old == -1 || old == b) break;
+        slot = (slot + 1) & (kHashCap - 1);
+        if (++probes == kHashCap) {
+          slot = -1;
+          break;
         }
       }
+      if (slot < 0) {
+        h.flag = 1;
+      } else {
+        if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
+        else atomicOr(&h.hi[slot], 1u << (bit - 32));
+      }
+    };
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t e = E0 + k * kWave + lane;
+      if (e < E1) insert(e, pcol[k]);
+    }
+    for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
+      const int32_t e = c + lane;
+      if (e < E1) insert(e, a.indices[e]);
     }
     wsync();
     if (h.flag) {
@@ -342,15 +388,21 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         o.nd[s] = nd;
       }
     } else {
-    // ---- occupied keys, compacted (keys into voff[], free until the kept blocks are numbered,
-    // slots into sslot[]), then ranked: sslot[] lists the slots in ascending key order
+    // ---- listed keys, compacted (keys into voff[], free until the kept blocks are numbered,
+    // slots into sslot[]), then ranked: sslot[] lists the slots in ascending key order. The
+    // list holds every occupied block when the reference-order block list is written, else
+    // only the kept ones (on remainder-heavy graphs a stripe has a few kept blocks among
+    // a hundred), whose voff[] entries are the only ones set >= 0 below.
+    const bool all_blocks = o.blk_bc != nullptr;
     int32_t D = 0;
 #pragma unroll
     for (int t = 0; t < kHashCap / kWave; ++t) {
       const int i = t * kWave + lane;
       const int32_t k = h.key[i];
-      const uint64_t m = __ballot(k >= 0);
-      if (k >= 0) {
+      const bool listed =
+          k >= 0 && (all_blocks || __popc(h.lo[i]) + __popc(h.hi[i]) >= a.min_nnz);
+      const uint64_t m = __ballot(listed);
+      if (listed) {
         const int pos = D + __popcll(m & below_lane);
         h.voff[pos] = k;
         h.sslot[pos] = i;
@@ -378,6 +430,8 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
         if (t * kWave + lane < D) h.sslot[rank[t]] = myslot[t];
+        // kept-only list: every unlisted block is rejected (the listed ones are set below)
+        if (!all_blocks) h.voff[t * kWave + lane] = -1;
       }
     }
     wsync();
@@ -385,9 +439,9 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     // ---- kept blocks: tile indices and value offsets by prefix sums over the sorted list
     int32_t tile_base = 0, val_base = 0, d_base = 0;
     if (FILL) {
-      tile_base = o.nt[s];
-      val_base = o.nv[s];
-      d_base = o.nd[s];
+      tile_base = cur.nt;
+      val_base = cur.nv;
+      d_base = cur.nd;
     }
     int32_t kcar = 0, vcar = 0;
     for (int j0 = 0; j0 < D; j0 += kWave) {
@@ -431,13 +485,11 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     wsync();
 
     // ---- entries: kept values into their tile, rejected ones to the remainder row
-    for (int32_t c = E0; c < E1; c += kWave) {
+    auto sweep = [&](int32_t c, int32_t col, V val) {
       const int32_t e = c + lane;
       const bool valid = e < E1;
       int r = 0, slot = 0, bit = 0;
-      int32_t col = 0;
       if (valid) {
-        col = a.indices[e];
         r = row_at(e);
         const int32_t b = col / bc;
         bit = r * bc + (col - b * bc);
@@ -453,13 +505,13 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       if constexpr (FILL) {
         if (valid) {
           if (rej) {
-            const int32_t pos = o.nr[r0 + r] + h.rcnt[r] + before;
+            const int32_t pos = h.rcnt[r] + before;
             o.rem_col[pos] = col;
-            ((V*)o.rem_val)[pos] = vals[e];
+            ((V*)o.rem_val)[pos] = val;
           } else {
             const uint64_t code = ((uint64_t)h.hi[slot] << 32) | h.lo[slot];
             const uint64_t below = bit == 0 ? 0ull : (~0ull >> (64 - bit));
-            ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = vals[e];
+            ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = val;
           }
         }
       }
@@ -469,6 +521,16 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       wsync();
       if (last) h.rcnt[r] += in_row;
       wsync();
+    };
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t c = E0 + k * kWave;
+      if (c < E1) sweep(c, pcol[k], FILL ? pval[k] : V(0));
+    }
+    for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
+      const int32_t e = c + lane;
+      const bool ok = e < E1;
+      sweep(c, ok ? a.indices[e] : 0, (FILL && ok) ? vals[e] : V(0));
     }
     if (!FILL && lane < br && r0 + lane < a.rows) o.nr[r0 + lane] = h.rcnt[lane];
     wsync();
