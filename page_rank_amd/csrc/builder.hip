@@ -245,7 +245,9 @@ __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
   return lo;
 }
 
-template <bool FILL, typename V>
+// P2: block_cols is a power of two, so an entry's block column and bit come from a shift and
+// a mask instead of an integer division (the kernel is bound by VALU issue).
+template <bool FILL, typename V, bool P2>
 __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     k_build_hash(BuildArgs a, FillOut o) {
   __shared__ HashLds<FILL> sh_all[4];
@@ -257,7 +259,13 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
   const V* __restrict__ vals = (const V*)o.values;
   const int br = a.br, bc = a.bc;
+  const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
   const uint64_t below_lane = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // block column of a column, and the entry's bit in its block's code (row r of the stripe)
+  auto block_of = [&](int32_t col) { return P2 ? (col >> bc_shift) : col / bc; };
+  auto bit_of = [&](int r, int32_t col, int32_t b) {
+    return P2 ? ((r << bc_shift) | (col & (bc - 1))) : r * bc + (col - b * bc);
+  };
 
   // Per-stripe loads that do not depend on the stripe's entries: CSR row starts (row k of the
   // stripe in lane k, row 64 in lane 0's second register) and, in the fill pass, the stripe's
@@ -324,10 +332,12 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     };
 
     // ---- insert every entry's block
-    auto insert = [&](int32_t e, int32_t col) {
+    // returns the entry's slot | bit << 8 | row << 14 (-1 if the table is full), which the
+    // entry sweep takes instead of finding the slot again
+    auto insert = [&](int32_t e, int32_t col) -> int32_t {
       const int r = row_at(e);
-      const int32_t b = col / bc;
-      const int bit = r * bc + (col - b * bc);
+      const int32_t b = block_of(col);
+      const int bit = bit_of(r, col, b);
       int slot = hash_slot(b);
       int probes = 0;
       for (;;) {
@@ -341,15 +351,17 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       }
       if (slot < 0) {
         h.flag = 1;
-      } else {
-        if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
-        else atomicOr(&h.hi[slot], 1u << (bit - 32));
+        return -1;
       }
+      if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
+      else atomicOr(&h.hi[slot], 1u << (bit - 32));
+      return slot | (bit << 8) | (r << 14);
     };
+    int32_t pinf[kHashPre];
 #pragma unroll
     for (int k = 0; k < kHashPre; ++k) {
       const int32_t e = E0 + k * kWave + lane;
-      if (e < E1) insert(e, pcol[k]);
+      pinf[k] = e < E1 ? insert(e, pcol[k]) : -1;
     }
     for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
       const int32_t e = c + lane;
@@ -363,7 +375,8 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     }
 
     if constexpr (!FILL) {
-      // counts do not depend on the tile order: sweep the table as it is
+      // counts do not depend on the tile order: sweep the table as it is (per-row remainder
+      // counts from the kept codes, with LDS atomics, measured slower than the entry sweep)
       int32_t nd = 0, nt = 0, nv = 0;
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
@@ -485,16 +498,23 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
     wsync();
 
     // ---- entries: kept values into their tile, rejected ones to the remainder row
-    auto sweep = [&](int32_t c, int32_t col, V val) {
+    // inf: what insert returned for the entry, or -1 to find its slot again
+    auto sweep = [&](int32_t c, int32_t col, V val, int32_t inf) {
       const int32_t e = c + lane;
       const bool valid = e < E1;
       int r = 0, slot = 0, bit = 0;
       if (valid) {
-        r = row_at(e);
-        const int32_t b = col / bc;
-        bit = r * bc + (col - b * bc);
-        slot = hash_slot(b);
-        while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
+        if (inf >= 0) {
+          slot = inf & 255;
+          bit = (inf >> 8) & 63;
+          r = inf >> 14;
+        } else {
+          r = row_at(e);
+          const int32_t b = block_of(col);
+          bit = bit_of(r, col, b);
+          slot = hash_slot(b);
+          while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
+        }
       }
       const bool rej = valid && h.voff[slot] < 0;
       const uint64_t rm = __ballot(rej);
@@ -525,12 +545,12 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
 #pragma unroll
     for (int k = 0; k < kHashPre; ++k) {
       const int32_t c = E0 + k * kWave;
-      if (c < E1) sweep(c, pcol[k], FILL ? pval[k] : V(0));
+      if (c < E1) sweep(c, pcol[k], FILL ? pval[k] : V(0), pinf[k]);
     }
     for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
       const int32_t e = c + lane;
       const bool ok = e < E1;
-      sweep(c, ok ? a.indices[e] : 0, (FILL && ok) ? vals[e] : V(0));
+      sweep(c, ok ? a.indices[e] : 0, (FILL && ok) ? vals[e] : V(0), -1);
     }
     if (!FILL && lane < br && r0 + lane < a.rows) o.nr[r0 + lane] = h.rcnt[lane];
     wsync();
@@ -665,25 +685,31 @@ static int group_lanes(int br) {
   return g;
 }
 
+template <bool FILL, typename V, bool P2>
+static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t st) {
+  // waves: the fill pass fills the CUs once (fewer or more workgroups were slower), the
+  // count pass uses kHashCountWgPerCu
+  static std::atomic<int> per_cu{-1};
+  if (per_cu.load() < 0) {
+    int n = kHashCountWgPerCu;
+    if (FILL &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_build_hash<FILL, V, P2>, 256, 0) !=
+             hipSuccess ||
+         n < 1))
+      n = 4;
+    per_cu.store(n);
+  }
+  const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
+  const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
+  hipLaunchKernelGGL((k_build_hash<FILL, V, P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
+                     st, a, o);
+}
+
 template <bool FILL, typename V>
 static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
   if (o.big != nullptr) {
-    // waves: the fill pass fills the CUs once (fewer or more workgroups were slower), the
-    // count pass uses kHashCountWgPerCu
-    static std::atomic<int> per_cu{-1};
-    if (per_cu.load() < 0) {
-      int n = kHashCountWgPerCu;
-      if (FILL &&
-          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_build_hash<FILL, V>, 256, 0) !=
-               hipSuccess ||
-           n < 1))
-        n = 4;
-      per_cu.store(n);
-    }
-    const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
-    const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
-    hipLaunchKernelGGL((k_build_hash<FILL, V>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
-                       st, a, o);
+    if ((a.bc & (a.bc - 1)) == 0) launch_build_hash<FILL, V, true>(a, o, st);
+    else launch_build_hash<FILL, V, false>(a, o, st);
     GL_LAUNCH_CHECK("k_build_hash");
   }
   const int GLn = group_lanes(a.br);
