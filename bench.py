@@ -54,6 +54,8 @@ def parse():
                     help="seed-set sizes are uniform in 1..this")
     ap.add_argument("--ppr-intra", type=float, default=0.8,
                     help="fraction of the PPR graph's edges inside their community")
+    ap.add_argument("--ppr-community", type=int, default=1000,
+                    help="nodes per community of the PPR graph")
     ap.add_argument("--ppr-profile-passes", type=int, default=0,
                     help="profiling aid: build the PPR graph, run this many plain passes of "
                          "the step (eager launches, no stop) twice, print their time, exit")
@@ -253,7 +255,7 @@ def ppr_leg(args, dev):
     from page_rank_amd import graphs
     n, m, K = args.ppr_nodes, args.ppr_edges, args.ppr_sets
     log(f"generating the {n} node / {m} edge clustered graph ...")
-    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // 1000),
+    src, dst = graphs.clustered_edges(n, m, communities=max(1, n // args.ppr_community),
                                       intra=args.ppr_intra, seed=3)
     log("  edges drawn; building the column-stochastic CSR ...")
     A, dang = graphs.transition_matrix(src, dst, n)
@@ -278,7 +280,8 @@ def ppr_leg(args, dev):
         tp, _ = _timed_runs(lambda: gl.run_ppr(pbr, sets, **pk), reps=1)
         log(f"  {np_} plain passes: {tp / np_ * 1e3:.3f} ms per pass")
         return {"profile_passes": np_, "ms_per_pass": round(tp / np_ * 1e3, 4),
-                "intra": args.ppr_intra, "nodes": n, "edges_nnz": int(A.nnz)}
+                "intra": args.ppr_intra, "community": args.ppr_community, "nodes": n,
+                "edges_nnz": int(A.nnz), "tiles": T, "remainder_nnz": R}
     tuned = {k: v for k, v in gl.autotune.tune_ppr(pbr, K).items() if k != "table"}
     log(f"  autotuned launch: {tuned}")
     kw = dict(alpha=0.85, max_iter=300, dangling=dang if dang.any() else None,
