@@ -431,6 +431,30 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// Interleaved form of load_row for a lane slice of N = 4 H floats (H >= 2) among L lanes of
+// a row: chunk h of lane q holds columns h * 4L + 4q .. +3, so the L lanes' h-th float4 loads
+// read one contiguous 16 L-byte span of the row (a single 128-byte line at 64 columns)
+// instead of every line of it. `valid`: columns of the row that may be read (0: none).
+template <int N, int L, bool BUF>
+__device__ __forceinline__ void load_row_il(const Ld<BUF>& xl, int64_t row0, int q, bool wide,
+                                            int valid, float (&out)[N]) {
+  static_assert(N % 4 == 0 && N >= 8, "interleaved rows take whole float4 chunks");
+#pragma unroll
+  for (int h = 0; h < N / 4; ++h) {
+    const int c = h * 4 * L + 4 * q;
+    if (wide) {
+      const float4 t = xl.f32x4(row0 + c, c < valid);
+      out[4 * h] = t.x;
+      out[4 * h + 1] = t.y;
+      out[4 * h + 2] = t.z;
+      out[4 * h + 3] = t.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[4 * h + j] = xl.f32(row0 + c + j, c + j < valid);
+    }
+  }
+}
+
 // Metadata of up to 64 tiles [t, t1), one tile per lane.
 struct TileMeta {
   uint32_t clo, chi;
@@ -532,6 +556,8 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   const int qq = lane % L;                // remainder: slot in the row
   const int kq = qq * FPL;
   const int nvq = kvalid - kq;
+  // slices of 8+ floats per lane are interleaved float4 chunks (load_row_il)
+  constexpr bool IL = FPL >= 8;
   const int gbase = lane & ~(L - 1);
   float* __restrict__ ra = racc[wid];
 
@@ -693,7 +719,10 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
             const int c = __shfl(scol, gbase + ib + i, kWave);
             a[i] = __shfl(sval, gbase + ib + i, kWave);
             const bool ok = jb + ib + i < rs.n();
-            load_row<FPL>(ld.xr, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
+            if constexpr (IL)
+              load_row_il<FPL, L>(ld.xr, xrow<BUF>(c, d.ldx), qq, vec, ok ? kvalid : 0, xv[i]);
+            else
+              load_row<FPL>(ld.xr, xrow<BUF>(c, d.ldx) + kq, vec, ok ? nvq : 0, xv[i]);
           }
 #pragma unroll
           for (int i = 0; i < EB; ++i)
@@ -701,8 +730,12 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
             for (int u = 0; u < FPL; ++u) racc2[u] = __builtin_fmaf(a[i], xv[i][u], racc2[u]);
         }
       }
-      // transpose through LDS: (row rr, columns kq..) -> (column f, all rows)
-      if (kq < KW) {
+      // transpose through LDS: (row rr, this lane's columns) -> (column f, all rows)
+      if constexpr (IL) {
+#pragma unroll
+        for (int u = 0; u < FPL; ++u)
+          ra[rr * KW + (u >> 2) * (4 * L) + 4 * qq + (u & 3)] = racc2[u];
+      } else if (kq < KW) {
 #pragma unroll
         for (int u = 0; u < FPL; ++u)
           if (kq + u < KW) ra[rr * KW + kq + u] = racc2[u];
