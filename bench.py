@@ -282,8 +282,13 @@ def ppr_leg(args, dev):
         return {"profile_passes": np_, "ms_per_pass": round(tp / np_ * 1e3, 4),
                 "intra": args.ppr_intra, "community": args.ppr_community, "nodes": n,
                 "edges_nnz": int(A.nnz), "tiles": T, "remainder_nnz": R}
-    tuned = {k: v for k, v in gl.autotune.tune_ppr(pbr, K).items() if k != "table"}
-    log(f"  autotuned launch: {tuned}")
+    # autotuned tiles (the 8x8 threshold) and launch (columns per launch, grid)
+    mn_best, pbr, tiles_tab = gl.autotune.tune_ppr_tiles(A, K, built={4: pbr})
+    tuned = {k: v for k, v in tiles_tab[mn_best].items() if k != "table"}
+    tuned["min_nnz_per_block"] = mn_best
+    tuned["ms_per_iter_by_min_nnz"] = {str(k): v["ms_per_iter"] for k, v in tiles_tab.items()}
+    T, V, R = pbr.accounted_blocks(), pbr.compressed_nnz(), pbr.remainder_nnz()
+    log(f"  autotuned launch: {tuned}; PBR 8x8 min {mn_best}: T={T} V={V} R={R}")
     kw = dict(alpha=0.85, max_iter=300, dangling=dang if dang.any() else None,
               dangling_mode=dmode, retire="per_column", return_history=False,
               kw=tuned["kw"], grid_mult=tuned["grid_mult"])

@@ -1,5 +1,5 @@
-"""Autotuner of the launch parameters: PBR block size, RHS columns per launch (kw), SpMM
-mode (fused / split) and the PPR grid size (workgroups per CU).
+"""Autotuner of the launch parameters: PBR block size and tile threshold, RHS columns per
+launch (kw), SpMM mode (fused / split) and the PPR grid size (workgroups per CU).
 
 Every choice is measured on the device with the real matrix and RHS width. The winner is
 cached in memory and in a JSON file, keyed by a shape signature: device name, block size,
@@ -151,6 +151,32 @@ def tune_ppr(pbr, K, *, reps=2, iters=4, force=False):
            "table": [[c[0], c[1], round(t / iters, 4)] for c, t in table]}
     cache().put(key, out)
     return out
+
+
+# PPR tile thresholds: 4 keeps every 8x8 block with 4+ entries as a tile; 16 and above turn
+# the sparse tiles of community graphs into remainder entries (a 64-column tile loads 8 full
+# X rows for its few values, two cache lines each, where a remainder entry loads one row)
+PPR_MIN_NNZ = (4, 8, 16)
+
+
+def tune_ppr_tiles(A_csr, K, *, block=8, min_nnz=PPR_MIN_NNZ, reps=2, iters=4, device=None,
+                   built=None):
+    """Build the matrix with each tile threshold (min_nnz_per_block) on the GPU, tune the PPR
+    step on each, and return (best threshold, its PBRMatrix, {threshold: tuned entry}).
+    `built` may map a threshold to a matrix already built from A_csr with that threshold and
+    block size, which is then used instead of building it again. The other matrices are
+    released before returning."""
+    from .builder import csr_to_pbr
+    built = dict(built or {})
+    results, mats = {}, {}
+    for mn in min_nnz:
+        pbr = built.get(mn)
+        if pbr is None:
+            pbr = csr_to_pbr(A_csr, block, block, mn, order="canonical", device=device)
+        results[mn] = tune_ppr(pbr, K, reps=reps, iters=iters)
+        mats[mn] = pbr
+    best = min(results, key=lambda mn: results[mn]["ms_per_iter"])
+    return best, mats[best], results
 
 
 def tune_block(A_csr, K, *, blocks=(2, 4, 8), min_nnz_per_block=None, reps=5, device=None):

@@ -73,3 +73,26 @@ def test_tuned_spmm_and_ppr_match_defaults(tmp_path, monkeypatch, device):
     assert np.abs(a.X.cpu().numpy() - b.X.cpu().numpy()).sum(0).max() < 1e-5
     best, res = autotune.tune_block(A, 8, blocks=(4, 8), device=device)
     assert best in (4, 8) and set(res) == {4, 8}
+
+
+@pytest.mark.gpu
+def test_ppr_tile_threshold_tuning(tmp_path, monkeypatch, device):
+    monkeypatch.setenv("PAGE_RANK_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_cache", None)
+    A = blocky_csr(3000, 8, 4, 2, 30, scatter=4000, seed=5)
+    Acs = A.multiply(1.0 / np.maximum(np.asarray(A.sum(0)).ravel(), 1e-30)).tocsr().astype(np.float32)
+    P4 = graph_link.csr_to_pbr(Acs, 8, 8, 4).to(device)
+    sets = [[i, i + 7] for i in range(0, 3000, 200)]
+    best, P, res = autotune.tune_ppr_tiles(Acs, len(sets), device=device, built={4: P4})
+    assert best in autotune.PPR_MIN_NNZ and set(res) == set(autotune.PPR_MIN_NNZ)
+    # the chosen matrix is the one built with that threshold: more entries leave the tiles
+    # as the threshold rises
+    tiles = {mn: graph_link.csr_to_pbr(Acs, 8, 8, mn).accounted_blocks()
+             for mn in autotune.PPR_MIN_NNZ}
+    assert tiles[4] >= tiles[8] >= tiles[16] and tiles[4] > tiles[16]
+    assert P.accounted_blocks() == tiles[best] and P.total_nnz == P4.total_nnz
+    # same power iteration on another split of the same matrix: sums in another order only
+    a = graph_link.run_ppr(P4, sets, tol=1e-7, max_iter=100)
+    b = graph_link.run_ppr(P, sets, tol=1e-7, max_iter=100)
+    assert np.abs(a.iterations - b.iterations).max() <= 1
+    assert np.abs(a.X.cpu().numpy() - b.X.cpu().numpy()).sum(0).max() < 1e-5
