@@ -241,19 +241,22 @@ int gl_ws_node_buckets(int64_t n, const int32_t* indeg, const int32_t* outdeg, i
                        void* stream);
 /* X0 from the table for pairs beyond hop 2; pairs within hop 2 appended to near (count in
  * *near_count on the device, list truncated at near_cap). node_buckets from
- * gl_ws_node_buckets. */
+ * gl_ws_node_buckets. x4 (may be NULL): also receives the 4th power-iteration iterate from
+ * the seed start, rebuilt from the feature products (v5 = A^j P, j = 0..4):
+ * x4 = (1 - alpha) (v0 + alpha v1 + alpha^2 v2 + alpha^3 v3) + alpha^4 v4. */
 int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
                    const int32_t* outdeg, const float* const* v5, const int32_t* node_buckets,
                    const float* table, float* x0, int64_t* near, int64_t near_cap,
-                   uint64_t* near_count, void* stream);
+                   uint64_t* near_count, float* x4, void* stream);
 /* x0[pairs[m]] = val[m]. */
 int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, void* stream);
-/* x0 <- max(0, x4 + lam (x0 - x4)), elementwise over [N][K], where x4 is the 4th
- * power-iteration iterate from the seed start rebuilt from the feature products
- * (v5 = A^j P, j = 0..4): the four products the features needed count as the first four
- * passes, and the network's start moves that iterate a fraction lam of the way. */
-int gl_ws_blend_iterate(float* x0, const float* const* v5, int64_t N, int64_t K, float alpha,
-                        float lam, void* stream);
+/* The blended start: x0 (the network's start, not normalized) <- max(0, x4 + lam (x0 / s - x4))
+ * with s its column sums and x4 from gl_ws_assemble, then every column scaled to sum 1: the
+ * four products the features needed count as the first four passes, and the network's start
+ * moves that iterate a fraction lam of the way. K <= 1024; sums: K * 1026 floats of scratch
+ * (the network start's column sums in [0, K), the result's in [K, 2K), fixed order). */
+int gl_ws_blend_start(float* x0, const float* x4, int64_t N, int64_t K, float lam, float* sums,
+                      void* stream);
 /* Scale every column of x [N][K] (K <= 1024) to sum 1. sums: K * 1025 floats of scratch; the
  * first K receive the column sums, taken in a fixed order (repeated calls agree bit for bit). */
 int gl_ws_normalize(float* x, int64_t N, int64_t K, float* sums, void* stream);

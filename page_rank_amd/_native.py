@@ -94,10 +94,10 @@ _PROTOS = {
     "gl_ws_table_features": (ctypes.c_int, [c_p, c_p]),
     "gl_ws_node_buckets": (ctypes.c_int, [c_i64, c_p, c_p, c_p, c_p]),
     "gl_ws_assemble": (ctypes.c_int, [c_i64, c_i64, c_f32, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
-                                      c_i64, c_p, c_p]),
+                                      c_i64, c_p, c_p, c_p]),
     "gl_ws_scatter": (ctypes.c_int, [c_p, c_p, c_i64, c_p, c_p]),
     "gl_ws_normalize": (ctypes.c_int, [c_p, c_i64, c_i64, c_p, c_p]),
-    "gl_ws_blend_iterate": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_f32, c_f32, c_p]),
+    "gl_ws_blend_start": (ctypes.c_int, [c_p, c_p, c_i64, c_i64, c_f32, c_p, c_p]),
     "gl_csr_normalize_workspace_bytes": (c_i64, [c_i64, c_i64]),
     "gl_csr_normalize_columns_f32": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p,
                                                     c_i64, c_p]),

@@ -311,11 +311,21 @@ __global__ void k_node_buckets(int64_t n, const int32_t* __restrict__ indeg,
 // near pairs in an LDS buffer and reserves their slots with one counter update per flush:
 // one update per row instead would serialize millions of atomics on the one counter.
 constexpr int kNearBuf = 512;  // pairs per wave buffer (>= 64: a row chunk always fits)
+// The 4th power-iteration iterate from the seed start, from the feature products
+// v_j = A^j P (j = 0..4): x4 = (1 - alpha) (v0 + alpha v1 + alpha^2 v2 + alpha^3 v3)
+// + alpha^4 v4.
+__device__ __forceinline__ float iterate4(float v0, float v1, float v2, float v3, float v4,
+                                          float alpha) {
+  const float a2 = alpha * alpha, a3 = a2 * alpha, a4 = a3 * alpha;
+  return (1.f - alpha) * (v0 + alpha * v1 + a2 * v2 + a3 * v3) + a4 * v4;
+}
+// x4 (may be null): also write the 4th iterate, from the products this kernel reads anyway.
 __global__ void __launch_bounds__(256) k_assemble(WsView s, const int32_t* __restrict__ nb,
                                                   const float* __restrict__ table,
                                                   float* __restrict__ x0,
                                                   int64_t* __restrict__ near, int64_t cap,
-                                                  unsigned long long* __restrict__ count) {
+                                                  unsigned long long* __restrict__ count,
+                                                  float* __restrict__ x4) {
   __shared__ int64_t buf_all[4][kNearBuf];
   const int lane = threadIdx.x & 63;
   int64_t* __restrict__ buf = buf_all[threadIdx.x >> 6];
@@ -340,10 +350,14 @@ __global__ void __launch_bounds__(256) k_assemble(WsView s, const int32_t* __res
       const bool ok = k < s.K;
       const int64_t ix = i * s.K + k;
       int hop = 5;
+      float vj[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
       if (ok) {
 #pragma unroll
-        for (int j = 4; j >= 0; --j)
-          if (s.v[j][ix] > 0.f) hop = j;
+        for (int j = 4; j >= 0; --j) {
+          vj[j] = s.v[j][ix];
+          if (vj[j] > 0.f) hop = j;
+        }
+        if (x4 != nullptr) x4[ix] = iterate4(vj[0], vj[1], vj[2], vj[3], vj[4], s.alpha);
       }
       const bool is_near = ok && hop <= 2;
       const uint64_t m = __ballot(is_near);
@@ -415,19 +429,38 @@ __global__ void __launch_bounds__(1024) k_colsum_final(const float* __restrict__
     sums[k] = t;
   }
 }
-// x0 <- max(0, x4 + lam (x0 - x4)) with x4 = the 4th power-iteration iterate from the seed
-// start, rebuilt from the feature products v_j = A^j P (j = 0..4):
-// x4 = (1 - alpha) (v0 + alpha v1 + alpha^2 v2 + alpha^3 v3) + alpha^4 v4.
-__global__ void __launch_bounds__(256) k_blend_iterate(float* __restrict__ x0, const float* v0,
-                                                       const float* v1, const float* v2,
-                                                       const float* v3, const float* v4,
-                                                       int64_t n, float alpha, float lam) {
-  const float c = 1.f - alpha, a2 = alpha * alpha, a3 = a2 * alpha, a4 = a3 * alpha;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const float x4 = c * (v0[e] + alpha * v1[e] + a2 * v2[e] + a3 * v3[e]) + a4 * v4[e];
-    x0[e] = fmaxf(0.f, x4 + lam * (x0[e] - x4));
+// x0 <- max(0, x4 + lam (x0 / msum - x4)) for the network's raw start x0 with column sums
+// msum and the 4th iterate x4, and the workgroups' partial column sums of the result (the
+// order of k_colsum, so k_colsum_final can add them): the network's start normalized, the
+// blend and the next column sums in one pass.
+__global__ void __launch_bounds__(256) k_blend_sums(float* __restrict__ x0,
+                                                    const float* __restrict__ x4, int64_t N,
+                                                    int64_t K, const float* __restrict__ msum,
+                                                    float lam, float* __restrict__ part) {
+  __shared__ float acc[4][1024];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k0 = 0; k0 < K; k0 += 64) {
+    const int64_t k = k0 + lane;
+    float a = 0.f;
+    if (k < K) {
+      const float sk = msum[k];
+      const float inv = sk > 0.f ? 1.f / sk : 0.f;
+      for (int64_t i = w0; i < N; i += nw) {
+        const int64_t e = i * K + k;
+        const float xi = x4[e];
+        const float y = fmaxf(0.f, xi + lam * (x0[e] * inv - xi));
+        x0[e] = y;
+        a += y;
+      }
+      acc[wid][k] = a;
+    }
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < K; k += blockDim.x)
+    part[blockIdx.x * K + k] = ((acc[0][k] + acc[1][k]) + acc[2][k]) + acc[3][k];
 }
 
 __global__ void k_colscale(float* __restrict__ x, int64_t N, int64_t K, const float* __restrict__ sums) {
@@ -568,13 +601,14 @@ int gl_ws_node_buckets(int64_t n, const int32_t* indeg, const int32_t* outdeg, i
 int gl_ws_assemble(int64_t N, int64_t K, float alpha, const int32_t* indeg,
                    const int32_t* outdeg, const float* const* v5, const int32_t* node_buckets,
                    const float* table, float* x0, int64_t* near, int64_t near_cap,
-                   uint64_t* near_count, void* stream) {
+                   uint64_t* near_count, float* x4, void* stream) {
   GL_REQUIRE(v5 && node_buckets && table && x0 && near && near_count, "null pointer");
   WsView s = make_view(N, K, alpha, indeg, outdeg, v5);
   hipStream_t st = (hipStream_t)stream;
   GL_HIP(hipMemsetAsync(near_count, 0, 8, st));
   hipLaunchKernelGGL(k_assemble, dim3(nblocks(N * 64, 256, 16384)), dim3(256), 0, st, s,
-                     node_buckets, table, x0, near, near_cap, (unsigned long long*)near_count);
+                     node_buckets, table, x0, near, near_cap, (unsigned long long*)near_count,
+                     x4);
   GL_LAUNCH_CHECK("k_assemble");
   return GL_OK;
 }
@@ -588,13 +622,27 @@ int gl_ws_scatter(const int64_t* pairs, const float* val, int64_t M, float* x0, 
   return GL_OK;
 }
 
-int gl_ws_blend_iterate(float* x0, const float* const* v, int64_t N, int64_t K, float alpha,
-                        float lam, void* stream) {
-  GL_REQUIRE(x0 && v && N >= 1 && K >= 1, "bad arguments");
-  for (int j = 0; j < 5; ++j) GL_REQUIRE(v[j] != nullptr, "feature product %d is null", j);
-  hipLaunchKernelGGL(k_blend_iterate, dim3(nblocks(N * K, 256, 16384)), dim3(256), 0,
-                     (hipStream_t)stream, x0, v[0], v[1], v[2], v[3], v[4], N * K, alpha, lam);
-  GL_LAUNCH_CHECK("k_blend_iterate");
+int gl_ws_blend_start(float* x0, const float* x4, int64_t N, int64_t K, float lam,
+                      float* sums, void* stream) {
+  GL_REQUIRE(x0 && x4 && sums && N >= 1 && K >= 1 && K <= 1024, "bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  // sums[0, K): the network start's column sums; sums[K, 2K): the result's;
+  // sums[2K, 2K + nb * K): the workgroups' partials of either
+  const int nb = nblocks(N * 64, 256, kColsumBlocks);
+  float* part = sums + 2 * K;
+  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, st, x0, N, K, part);
+  GL_LAUNCH_CHECK("k_colsum");
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((K + 63) / 64)), dim3(1024), 0, st, part,
+                     nb, K, sums);
+  GL_LAUNCH_CHECK("k_colsum_final");
+  hipLaunchKernelGGL(k_blend_sums, dim3(nb), dim3(256), 0, st, x0, x4, N, K, sums, lam, part);
+  GL_LAUNCH_CHECK("k_blend_sums");
+  hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((K + 63) / 64)), dim3(1024), 0, st, part,
+                     nb, K, sums + K);
+  GL_LAUNCH_CHECK("k_colsum_final");
+  hipLaunchKernelGGL(k_colscale, dim3(nblocks(N * 64, 256, 16384)), dim3(256), 0, st, x0, N, K,
+                     sums + K);
+  GL_LAUNCH_CHECK("k_colscale");
   return GL_OK;
 }
 

@@ -146,7 +146,7 @@ def pair_features(wi: WsInputs, pairs):
 
 
 # Weight of the network's start against the 4th power iterate that its feature products
-# already hold (gl_ws_blend_iterate). Measured on the 4M-node clustered graph (64 seed sets,
+# already hold (gl_ws_blend_start). Measured on the 4M-node clustered graph (64 seed sets,
 # tol 1.5e-7): lam 0 (the iterate alone) 41 passes after the 4 products, 0.25 37, 0.5 42,
 # 0.75 44, 1 (the network alone, no products counted) 45.
 BLEND = 0.25
@@ -164,13 +164,15 @@ def predict(params, wi: WsInputs, near_cap=None, blend=BLEND):
     _native.check(L.gl_ws_table_features(tfeat.data_ptr(), st), "gl_ws_table_features")
     table = forward(params, tfeat)
     x0 = torch.empty((wi.N, wi.K), dtype=torch.float32, device=dev)
+    # with a blend, the 4th iterate comes out of the same pass over the feature products
+    x4 = torch.empty_like(x0) if blend is not None else None
     cap = near_cap if near_cap is not None else max(1 << 16, wi.N * wi.K // 4)
     near = torch.empty(cap, dtype=torch.int64, device=dev)
     cnt = torch.zeros(1, dtype=torch.int64, device=dev)
     _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
                                    wi.outdeg.data_ptr(), wi.vptrs(), wi.nb.data_ptr(),
                                    table.data_ptr(), x0.data_ptr(), near.data_ptr(), cap,
-                                   cnt.data_ptr(), st),
+                                   cnt.data_ptr(), None if x4 is None else x4.data_ptr(), st),
                   "gl_ws_assemble")
     n_near = int(cnt.item())
     if n_near > cap:
@@ -180,15 +182,16 @@ def predict(params, wi: WsInputs, near_cap=None, blend=BLEND):
         vals = forward(params, pair_features(wi, pairs))
         _native.check(L.gl_ws_scatter(pairs.data_ptr(), vals.data_ptr(), n_near, x0.data_ptr(), st),
                       "gl_ws_scatter")
-    # the column sums, then the workgroups' partials (at most 1024 of K floats)
-    sums = torch.empty(wi.K * 1025, dtype=torch.float32, device=dev)
-    _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
-                  "gl_ws_normalize")
-    if blend is not None:
-        _native.check(L.gl_ws_blend_iterate(x0.data_ptr(), wi.vptrs(), wi.N, wi.K, wi.alpha,
-                                            float(blend), st), "gl_ws_blend_iterate")
+    if blend is None:
+        # the column sums, then the workgroups' partials (at most 1024 of K floats)
+        sums = torch.empty(wi.K * 1025, dtype=torch.float32, device=dev)
         _native.check(L.gl_ws_normalize(x0.data_ptr(), wi.N, wi.K, sums.data_ptr(), st),
                       "gl_ws_normalize")
+    else:
+        sums = torch.empty(wi.K * 1026, dtype=torch.float32, device=dev)
+        _native.check(L.gl_ws_blend_start(x0.data_ptr(), x4.data_ptr(), wi.N, wi.K,
+                                          float(blend), sums.data_ptr(), st),
+                      "gl_ws_blend_start")
     return x0, n_near
 
 
@@ -222,7 +225,7 @@ def training_pairs(wi: WsInputs, x_star: torch.Tensor, far_samples: int, seed=0,
     _native.check(L.gl_ws_assemble(wi.N, wi.K, wi.alpha, wi.indeg.data_ptr(),
                                    wi.outdeg.data_ptr(), wi.vptrs(), wi.nb.data_ptr(),
                                    dummy_table.data_ptr(), x0.data_ptr(), near.data_ptr(), cap,
-                                   cnt.data_ptr(), st),
+                                   cnt.data_ptr(), None, st),
                   "gl_ws_assemble")
     n_near = min(int(cnt.item()), cap)
     del x0
