@@ -73,14 +73,16 @@ def _seed_arrays(seed_sets, weights=None):
 def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=None,
             dangling_mode="none", retire="per_column", x0=None, weights=None,
             graph_iters=8, kw=0, grid_mult=0, extrapolate_every=0,
-            return_history=True) -> PPRResult:
+            return_history=True, x0_in_place=False) -> PPRResult:
     """Batched PPR, one column per seed set (a list of node-index lists).
     kw / grid_mult: RHS columns per launch and workgroups per CU (0 = library rule;
     kw="auto" = measured by the autotuner, cached per shape).
     extrapolate_every: 0 = plain power iteration; P >= 2 = after every P passes, columns whose
     residual shrinks by a steady ratio rho in [0.5, 0.97] jump ahead along their last update,
     X += rho / (1 - rho) (X_t - X_{t-1}) (Aitken extrapolation of the dominant error mode).
-    The stopping rule is unchanged, so the result meets the same residual bound."""
+    The stopping rule is unchanged, so the result meets the same residual bound.
+    x0_in_place: x0 (a contiguous float32 [N][K] tensor on the matrix's device) becomes the
+    result's buffer, with no copy; the caller's start is overwritten."""
     meta = get_pbr_gpu_meta(pbr_mat)
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
@@ -108,10 +110,18 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
             raise ValueError(f"dangling_mode={dangling_mode!r} needs the dangling mask")
         dmask = torch.as_tensor(np.asarray(dangling, dtype=np.uint8)).to(device)
     if x0 is not None:
+        given = x0
         x0 = torch.as_tensor(x0, dtype=torch.float32, device=device).contiguous()
         if x0.shape != (N, K):
             raise ValueError(f"x0 must be [{N}, {K}]")
-    X = torch.empty((N, K), dtype=torch.float32, device=device)
+        if x0_in_place and x0 is not given:
+            raise ValueError("x0_in_place needs a contiguous float32 tensor on the device")
+    if x0_in_place:
+        if x0 is None:
+            raise ValueError("x0_in_place needs x0")
+        X = x0  # the library runs in place when the start is the output buffer
+    else:
+        X = torch.empty((N, K), dtype=torch.float32, device=device)
     iters = torch.zeros(K, dtype=torch.int32, device=device)
     conv = torch.zeros(K, dtype=torch.int32, device=device)
     hist = (torch.zeros((max_iter, K), dtype=torch.float32, device=device)

@@ -58,7 +58,6 @@ class WsInputs:
     outdeg: torch.Tensor
     nb: torch.Tensor                        # table row of every node
     v: list = field(default_factory=list)  # A^j P, j = 0..4, each [N][K]
-    ms: float = 0.0
 
     def vptrs(self):
         return _ptrs(self.v)
@@ -113,17 +112,13 @@ def inputs(pbr, seed_sets, alpha=0.85, graph=None) -> WsInputs:
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
     dev = meta["codes"].device
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
     indeg, outdeg, nb = graph_features(pbr) if graph is None else graph
     v = [seed_matrix(pbr.rows, seed_sets, dev)]
     for _ in range(4):
         y = torch.empty_like(v[0])
         spmm_into(pbr, v[-1], y)
         v.append(y)
-    torch.cuda.synchronize(dev)
-    return WsInputs(pbr.rows, len(seed_sets), alpha, indeg, outdeg, nb, v,
-                    (time.perf_counter() - t0) * 1e3)
+    return WsInputs(pbr.rows, len(seed_sets), alpha, indeg, outdeg, nb, v)
 
 
 def forward(params, feat):
@@ -203,7 +198,7 @@ def run_warm(params, pbr, seed_sets, *, graph=None, alpha=0.85, blend=BLEND, **r
     wi = inputs(pbr, seed_sets, alpha, graph=graph)
     x0, _ = predict(params, wi, blend=blend)
     del wi
-    res = run_ppr(pbr, seed_sets, alpha=alpha, x0=x0, **run_kw)
+    res = run_ppr(pbr, seed_sets, alpha=alpha, x0=x0, x0_in_place=True, **run_kw)
     res.feature_passes = 4 if blend is not None else 0
     return res
 

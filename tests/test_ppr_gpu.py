@@ -257,6 +257,30 @@ def test_repeat_runs_are_bitwise_identical(device):
         assert np.array_equal(np.asarray(a.history), np.asarray(b.history))
 
 
+def test_start_vector_in_place(device):
+    """x0_in_place: the device start vector becomes the result's buffer (no copy) and the run
+    is the copying run bit for bit; a start that is not a contiguous float32 device tensor
+    is refused."""
+    n = 900
+    adj = _sbm(n, 90, 0.08, 0.004, seed=6)
+    A, dangling = oppr.column_stochastic(sp.csr_matrix(adj.T))
+    pbr = graph_link.csr_to_pbr(sp.csr_matrix(A, dtype=np.float32), 8, 8, 2).to(device)
+    rng = np.random.default_rng(6)
+    sets = [rng.choice(n, size=int(rng.integers(1, 12)), replace=False) for _ in range(24)]
+    x0 = rng.random((n, 24)).astype(np.float32)
+    x0 /= x0.sum(axis=0, keepdims=True)
+    kw = dict(tol=1e-6, max_iter=100, dangling=dangling, dangling_mode="uniform")
+    a = graph_link.run_ppr(pbr, sets, x0=x0, **kw)
+    xd = torch.from_numpy(x0).to(device)
+    b = graph_link.run_ppr(pbr, sets, x0=xd, x0_in_place=True, **kw)
+    assert b.X.data_ptr() == xd.data_ptr()
+    assert torch.equal(a.X, b.X) and np.array_equal(a.iterations, b.iterations)
+    with pytest.raises(ValueError):
+        graph_link.run_ppr(pbr, sets, x0=x0, x0_in_place=True, **kw)   # host array
+    with pytest.raises(ValueError):
+        graph_link.run_ppr(pbr, sets, x0_in_place=True, **kw)          # no start
+
+
 @pytest.mark.parametrize("every", [6, 10])
 def test_extrapolation_reaches_the_same_fixed_point(every, device):
     """Dominant-mode extrapolation (run_ppr extrapolate_every) on a community-clustered graph
