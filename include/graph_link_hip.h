@@ -77,6 +77,31 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
                         float* Y, int64_t ldy, int32_t accumulate, int32_t mode, int32_t kw,
                         void* stream);
 
+/* Execution layout of an 8x8 PBR matrix, derived once (at upload) from the reference's
+ * arrays and passed beside them to gl_pbr_spmm_exec_i32_f32. No counterpart in the
+ * reference: the same matrix, re-encoded for the matrix-core tile kernel.
+ *   tile_records: gl_pbr_exec_layout_bytes(num_tiles) bytes (16 per tile plus 8 padding
+ *                 records): the tile's code with its 8x8 bit matrix transposed, its first X
+ *                 row and its value offset;
+ *   values_cm:    num_vals floats, each tile's values in column-major bit order. */
+int64_t gl_pbr_exec_layout_bytes(int64_t num_tiles);
+int gl_pbr_exec_layout(int64_t num_tiles, int64_t num_vals, const uint64_t* codes,
+                       const int32_t* coords, const int32_t* offsets, const float* data,
+                       void* tile_records, float* values_cm, void* stream);
+
+/* gl_pbr_spmm_i32_f32 with the execution layout (both pointers null: the same as
+ * gl_pbr_spmm_i32_f32). With 8x8 tiles, fused mode and 32 RHS columns per launch, the tiles
+ * are multiplied on the matrix cores from the layout; every other case ignores it. */
+int gl_pbr_spmm_exec_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
+                             int32_t block_cols, int64_t num_tiles, int64_t num_vals,
+                             int64_t num_rem, const uint64_t* codes, const int32_t* coords,
+                             const int32_t* offsets, const float* data,
+                             const int32_t* stripe_ptr, const int32_t* rem_indptr,
+                             const int32_t* rem_col_ind, const float* rem_vals,
+                             const void* tile_records, const float* values_cm, const float* X,
+                             int64_t ldx, int64_t K, float* Y, int64_t ldy, int32_t accumulate,
+                             int32_t mode, int32_t kw, void* stream);
+
 /* Block-tile part alone. Rows with no tile get 0 (or keep Y). Replaces launch_pbr_spmm
  * (Backend/graph_link/kernels/pbr_kernel.cu:336-396). */
 int gl_pbr_block_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,

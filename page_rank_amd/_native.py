@@ -55,6 +55,12 @@ _PROTOS = {
                                            c_p, c_p, c_p, c_p,
                                            c_p, c_p, c_p, c_p, c_p, c_i64, c_i64, c_p, c_i64,
                                            c_i32, c_i32, c_i32, c_p]),
+    "gl_pbr_exec_layout_bytes": (c_i64, [c_i64]),
+    "gl_pbr_exec_layout": (ctypes.c_int, [c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "gl_pbr_spmm_exec_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64,
+                                                c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p,
+                                                c_p, c_p, c_p, c_i64, c_i64, c_p, c_i64,
+                                                c_i32, c_i32, c_i32, c_p]),
     "gl_pbr_block_spmm_i32_f32": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i64, c_i64,
                                                  c_p, c_p,
                                                  c_p, c_p, c_p, c_p, c_i64, c_i64, c_p, c_i64,

@@ -9,6 +9,7 @@
 #include "gl_common.hpp"
 #include "launch.hpp"
 #include "stripe_kernels.hpp"
+#include "stripe_mfma.hpp"
 
 namespace gl {
 
@@ -144,6 +145,36 @@ __global__ void k_stripe_index(int64_t num_stripes, int32_t B, int64_t T,
   }
 }
 
+// Execution layout of an 8x8 PBR matrix: one wave per tile, lane l = column-major
+// position 8c + r. The transposed code is the wave's ballot of "bit (8r + c) is set"; each
+// value moves from its row-major rank to its column-major rank inside the tile's range.
+__global__ void k_exec_layout(int64_t T, const uint64_t* __restrict__ codes,
+                              const int32_t* __restrict__ coords,
+                              const int32_t* __restrict__ offsets, const float* __restrict__ data,
+                              TileRec* __restrict__ trec, float* __restrict__ tdata) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  for (int64_t t = w0; t < T + kTileRecPad; t += waves) {
+    if (t >= T) {
+      if (lane == 0) trec[t] = TileRec{0u, 0u, 0, T > 0 ? offsets[T] : 0};
+      continue;
+    }
+    const uint64_t code = codes[t];
+    const int r = lane & 7, c = lane >> 3;
+    const int pos = 8 * r + c;
+    const bool has = (code >> pos) & 1ull;
+    const uint64_t tcode = __ballot(has);
+    const int off = offsets[t];
+    if (has) {
+      const int rm = __popcll(code & ((1ull << pos) - 1ull));
+      const int cm = __popcll(tcode & ((1ull << lane) - 1ull));
+      tdata[off + cm] = data[off + rm];
+    }
+    if (lane == 0) trec[t] = TileRec{(uint32_t)tcode, (uint32_t)(tcode >> 32), coords[2 * t + 1], off};
+  }
+}
+
 }  // namespace gl
 
 using namespace gl;
@@ -152,7 +183,7 @@ extern "C" {
 
 const char* gl_last_error(void) { return t_last_error.c_str(); }
 
-int32_t gl_abi_version(void) { return 7; }
+int32_t gl_abi_version(void) { return 8; }
 
 int gl_device_info(int32_t* num_cus, int32_t* lds_bytes_per_cu, int32_t* clock_khz,
                    int64_t* hbm_bytes) {
@@ -254,6 +285,24 @@ int gl_csr_spmm_i32_f32(int64_t rows, int64_t cols, int64_t nnz, const int32_t* 
   return GL_OK;
 }
 
+int64_t gl_pbr_exec_layout_bytes(int64_t num_tiles) {
+  return num_tiles < 0 ? 0 : (num_tiles + kTileRecPad) * (int64_t)sizeof(TileRec);
+}
+
+int gl_pbr_exec_layout(int64_t num_tiles, int64_t num_vals, const uint64_t* codes,
+                       const int32_t* coords, const int32_t* offsets, const float* data,
+                       void* tile_records, float* values_cm, void* stream) {
+  GL_REQUIRE(num_tiles >= 0 && num_vals >= 0, "negative size");
+  GL_REQUIRE(tile_records != nullptr && offsets != nullptr, "null tile records or offsets");
+  GL_REQUIRE(num_tiles == 0 || (codes && coords && data && values_cm), "null tile array");
+  const int64_t waves = num_tiles + kTileRecPad;
+  const int64_t blocks = std::min<int64_t>((waves + 3) / 4, 65536);
+  hipLaunchKernelGGL(k_exec_layout, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     num_tiles, codes, coords, offsets, data, (TileRec*)tile_records, values_cm);
+  GL_LAUNCH_CHECK("k_exec_layout");
+  return GL_OK;
+}
+
 int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
                         int64_t num_tiles, int64_t num_vals, int64_t num_rem,
                         const uint64_t* codes, const int32_t* coords,
@@ -262,6 +311,21 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
                         const float* rem_vals, const float* X, int64_t ldx, int64_t K,
                         float* Y, int64_t ldy, int32_t accumulate, int32_t mode, int32_t kw,
                         void* stream) {
+  return gl_pbr_spmm_exec_i32_f32(rows, cols, block_rows, block_cols, num_tiles, num_vals,
+                                  num_rem, codes, coords, offsets, data, stripe_ptr, rem_indptr,
+                                  rem_col_ind, rem_vals, nullptr, nullptr, X, ldx, K, Y, ldy,
+                                  accumulate, mode, kw, stream);
+}
+
+int gl_pbr_spmm_exec_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
+                             int32_t block_cols, int64_t num_tiles, int64_t num_vals,
+                             int64_t num_rem, const uint64_t* codes, const int32_t* coords,
+                             const int32_t* offsets, const float* data,
+                             const int32_t* stripe_ptr, const int32_t* rem_indptr,
+                             const int32_t* rem_col_ind, const float* rem_vals,
+                             const void* tile_records, const float* values_cm, const float* X,
+                             int64_t ldx, int64_t K, float* Y, int64_t ldy, int32_t accumulate,
+                             int32_t mode, int32_t kw, void* stream) {
   int rc = check_pbr(rows, cols, block_rows, block_cols, num_tiles, codes, coords, offsets,
                      data, stripe_ptr);
   if (rc) return rc;
@@ -287,6 +351,10 @@ int gl_pbr_spmm_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t 
   }
   PbrView m{rows, cols, (rows + block_rows - 1) / block_rows, num_tiles, num_vals, num_rem,
             codes, coords, offsets, data, stripe_ptr, rem_indptr, rem_col_ind, rem_vals};
+  if (block_rows == 8) {
+    m.trec = tile_records;
+    m.tdata = values_cm;
+  }
   const int nwg = grid_for(m.num_stripes, kSpmmWgPerCu);
   for (int64_t k0 = 0, w = kw; k0 < K; k0 += w) {
     w = chunk_kw(kw, K - k0);

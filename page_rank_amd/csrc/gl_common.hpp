@@ -108,4 +108,14 @@ __device__ __forceinline__ float shfl_xor(float v, int m) {
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// Wave-uniform read of an array that no kernel of the launch writes, through the scalar
+// cache (s_load). A plain load of a uniform address compiles to a vector load and a
+// readfirstlane, and the wait before that readfirstlane drains every vector memory operation
+// the wave has in flight (prefetches and stores alike), once per stripe in the stripe loops.
+// The constant address space tells the compiler the data cannot change during the kernel.
+template <class T>
+__device__ __forceinline__ T sload(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 }  // namespace gl

@@ -3,6 +3,7 @@
 
 #include "gl_common.hpp"
 #include "stripe_kernels.hpp"
+#include "stripe_mfma.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -81,6 +82,16 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
   // accumulators of columns that are not stored.
   const bool full = std::min<int64_t>(kw, d.K - d.k0) == kw;
   const bool vec = BUF || ((d.ldx & 3) == 0 && full);
+  if constexpr (B == 8 && BUF && TILES && std::is_same_v<Epi, EpiStore>) {
+    if (kw == 32 && m.trec != nullptr && m.tdata != nullptr) {
+      const bool aligned = (d.ldx & 3) == 0 && (d.k0 & 3) == 0 &&
+                           (reinterpret_cast<uintptr_t>(d.x) & 15u) == 0;
+      if (aligned) GL_GO((k_stripe_mx32<TILES, REM, true>));
+      else GL_GO((k_stripe_mx32<TILES, REM, false>));
+      if (!occ) GL_LAUNCH_CHECK("k_stripe_mx32");
+      return GL_OK;
+    }
+  }
   switch (kw) {
 #define GL_KB(KW)                                                                        \
     case KW:                                                                             \

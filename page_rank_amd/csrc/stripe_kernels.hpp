@@ -56,6 +56,9 @@ struct PbrView {
   const int32_t* __restrict__ rem_ptr;
   const int32_t* __restrict__ rem_col;
   const float* __restrict__ rem_val;
+  // execution layout of 8x8 matrices (tile records and column-major values), or null
+  const void* __restrict__ trec = nullptr;
+  const float* __restrict__ tdata = nullptr;
 };
 
 struct RhsView {
@@ -281,8 +284,8 @@ struct EpiPpr {
     tev = ok ? tele[col] : 0.f;
   }
   __device__ __forceinline__ void begin_stripe(int64_t s) {
-    e0 = sptr[s];
-    e1 = sptr[s + 1];
+    e0 = sload(sptr, s);
+    e1 = sload(sptr, s + 1);
   }
   // Element offset of (row, col) in X_t / X_{t+1}. With BUF (the buffer-descriptor
   // kernels, whose launch guarantees arrays below 2 GB and rows, ld below 2^24) it is a
@@ -638,12 +641,12 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
   int64_t s = sc.first;
   int ta0 = 0, ta1 = 0, tb0 = 0, tb1 = 0;
   if (TILES && s < sc.end) {
-    ta0 = m.stripe_ptr[s];
-    ta1 = m.stripe_ptr[s + 1];
+    ta0 = sload(m.stripe_ptr, s);
+    ta1 = sload(m.stripe_ptr, s + 1);
   }
   if (TILES && s + sc.stride < sc.end) {
-    tb0 = m.stripe_ptr[s + sc.stride];
-    tb1 = m.stripe_ptr[s + sc.stride + 1];
+    tb0 = sload(m.stripe_ptr, s + sc.stride);
+    tb1 = sload(m.stripe_ptr, s + sc.stride + 1);
   }
   TileMeta mt{0u, 0u, 0, 0};
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
@@ -664,8 +667,8 @@ k_stripe_feat(PbrView m, RhsView d, Epi epi) {
     RowSpan rsn{0, 0};
     if (TILES) mtn = load_tile_meta(ld, tb0, tb1, lane);
     if (TILES && snn < sc.end) {
-      tc0 = m.stripe_ptr[snn];
-      tc1 = m.stripe_ptr[snn + 1];
+      tc0 = sload(m.stripe_ptr, snn);
+      tc1 = sload(m.stripe_ptr, snn + 1);
     }
     if constexpr (REM) rsn = load_row_span(ld, m, sn * B + rr, sn < sc.end);
     // ---- first remainder entries of this stripe
@@ -843,12 +846,12 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
   int64_t s = sc.first;
   int ta0 = 0, ta1 = 0, tb0 = 0, tb1 = 0;
   if (TILES && s < sc.end) {
-    ta0 = m.stripe_ptr[s];
-    ta1 = m.stripe_ptr[s + 1];
+    ta0 = sload(m.stripe_ptr, s);
+    ta1 = sload(m.stripe_ptr, s + 1);
   }
   if (TILES && s + sc.stride < sc.end) {
-    tb0 = m.stripe_ptr[s + sc.stride];
-    tb1 = m.stripe_ptr[s + sc.stride + 1];
+    tb0 = sload(m.stripe_ptr, s + sc.stride);
+    tb1 = sload(m.stripe_ptr, s + sc.stride + 1);
   }
   TileMeta mt{0u, 0u, 0, 0};
   if (TILES) mt = load_tile_meta(ld, ta0, ta1, lane);
@@ -866,8 +869,8 @@ __global__ void __launch_bounds__(256) k_stripe_bit(PbrView m, RhsView d, Epi ep
     RowSpan rsn{0, 0};
     if (TILES) mtn = load_tile_meta(ld, tb0, tb1, lane);
     if (TILES && snn < sc.end) {
-      tc0 = m.stripe_ptr[snn];
-      tc1 = m.stripe_ptr[snn + 1];
+      tc0 = sload(m.stripe_ptr, snn);
+      tc1 = sload(m.stripe_ptr, snn + 1);
     }
     if constexpr (REM) rsn = load_row_span<KW == 1>(ld, m, sn * B + rl, sn < sc.end);
     // first remainder entry of this lane's row slot

@@ -6,6 +6,7 @@ import numpy as np
 import torch
 
 from . import _native
+from .pbr import exec_layout
 from .registry import get_pbr_gpu_meta
 
 
@@ -41,15 +42,18 @@ def spmm_into(pbr_mat, x: torch.Tensor, y: torch.Tensor, *, accumulate=False, mo
     L = _native.lib()
     m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
     with torch.cuda.device(x.device):
-        rc = L.gl_pbr_spmm_i32_f32(
+        lay = exec_layout(meta, pbr_mat.block_rows)
+        recs, vals = (lay[0].data_ptr(), lay[1].data_ptr()) if lay else (None, None)
+        rc = L.gl_pbr_spmm_exec_i32_f32(
             pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
             meta["codes"].numel(), meta["data"].numel(), meta["rem_col_ind"].numel(),
             meta["codes"].data_ptr(), meta["coords"].data_ptr(),
             meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
             meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
-            meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), K, y.data_ptr(),
-            y.stride(0), 1 if accumulate else 0, m, kw, _native.stream_handle(x.device))
-    _native.check(rc, "gl_pbr_spmm_i32_f32")
+            meta["rem_vals"].data_ptr(), recs, vals, x.data_ptr(), x.stride(0), K,
+            y.data_ptr(), y.stride(0), 1 if accumulate else 0, m, kw,
+            _native.stream_handle(x.device))
+    _native.check(rc, "gl_pbr_spmm_exec_i32_f32")
     return y
 
 
@@ -192,8 +196,11 @@ class SpmmPlan:
             from .autotune import tune_spmm
             t = tune_spmm(pbr_mat, x.shape[1])
             kw, mode = t["kw"], t["mode"]
-        self._keep = (meta, x, y)
-        self._fn = _native.lib().gl_pbr_spmm_i32_f32
+        with torch.cuda.device(x.device):
+            lay = exec_layout(meta, pbr_mat.block_rows)
+        recs, vals = (lay[0].data_ptr(), lay[1].data_ptr()) if lay else (None, None)
+        self._keep = (meta, x, y, lay)
+        self._fn = _native.lib().gl_pbr_spmm_exec_i32_f32
         m = {"fused": _native.GL_SPMM_FUSED, "split": _native.GL_SPMM_SPLIT}[mode]
         self._args = (
             pbr_mat.rows, pbr_mat.cols, pbr_mat.block_rows, pbr_mat.block_cols,
@@ -201,10 +208,11 @@ class SpmmPlan:
             meta["codes"].data_ptr(), meta["coords"].data_ptr(),
             meta["offsets"].data_ptr(), meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
             meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
-            meta["rem_vals"].data_ptr(), x.data_ptr(), x.stride(0), x.shape[1], y.data_ptr(),
-            y.stride(0), 1 if accumulate else 0, m, kw, _native.stream_handle(x.device))
+            meta["rem_vals"].data_ptr(), recs, vals, x.data_ptr(), x.stride(0), x.shape[1],
+            y.data_ptr(), y.stride(0), 1 if accumulate else 0, m, kw,
+            _native.stream_handle(x.device))
 
     def run(self):
         rc = self._fn(*self._args)
         if rc:
-            _native.check(rc, "gl_pbr_spmm_i32_f32")
+            _native.check(rc, "gl_pbr_spmm_exec_i32_f32")

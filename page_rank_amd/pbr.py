@@ -226,6 +226,31 @@ def stripe_index(coords_dev, rows, block_rows, num_tiles, device):
     return sp
 
 
+def exec_layout(meta, block_rows):
+    """The matrix-core SpMM's execution layout of an 8x8 float32 matrix (tile records with
+    transposed codes, values in column-major bit order), built on the device at the first
+    product and kept in the device arrays; None for other matrices."""
+    import torch
+    if block_rows != 8 or meta["data"].dtype != torch.float32:
+        return None
+    if "tile_records" not in meta:
+        L = _native.lib()
+        dev = meta["codes"].device
+        T, V = meta["codes"].numel(), meta["data"].numel()
+        recs = torch.empty(L.gl_pbr_exec_layout_bytes(T), dtype=torch.uint8, device=dev)
+        vals = torch.empty(max(V, 1), dtype=torch.float32, device=dev)
+        with torch.cuda.device(dev):
+            _native.check(L.gl_pbr_exec_layout(T, V, _native.ptr(meta["codes"]),
+                                               _native.ptr(meta["coords"]),
+                                               _native.ptr(meta["offsets"]),
+                                               _native.ptr(meta["data"]), _native.ptr(recs),
+                                               _native.ptr(vals), _native.stream_handle(dev)),
+                          "gl_pbr_exec_layout")
+        meta["values_cm"] = vals
+        meta["tile_records"] = recs
+    return meta["tile_records"], meta["values_cm"]
+
+
 def upload(p: PBRMatrix, device):
     """Copy host arrays to the device: codes as int64 bit patterns, all indices int32
     (as __init__.py:43-51), values as stored, plus the stripe index."""

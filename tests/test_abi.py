@@ -40,7 +40,7 @@ def test_ctypes_table_matches_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 7
+    assert L.gl_abi_version() == 8
     assert L.gl_last_error() is not None
 
 
