@@ -81,6 +81,8 @@ constexpr int kMxBatch = 8;  // tiles per batch
 
 typedef __attribute__((address_space(4))) const TileRec c_TileRec;
 
+
+
 // Largest value of v over the wave, with DPP and permlane swaps (no LDS round trips).
 __device__ __forceinline__ int wave_max_fast(int v) {
   v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));   // lane ^ 1
@@ -299,7 +301,9 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
           o.z += y0.z;
           o.w += y0.w;
         }
-        *reinterpret_cast<float4*>(yp) = o;
+        // non-temporal: Y is not read again by this launch, and its lines would otherwise
+        // push tile X rows out of L2 (measured 200 -> 189 us at 1M rows, 32 columns)
+        __builtin_nontemporal_store(v4f{o.x, o.y, o.z, o.w}, reinterpret_cast<v4f*>(yp));
       } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
