@@ -90,8 +90,8 @@ int gl_pbr_exec_layout(int64_t num_tiles, int64_t num_vals, const uint64_t* code
                        void* tile_records, float* values_cm, void* stream);
 
 /* gl_pbr_spmm_i32_f32 with the execution layout (both pointers null: the same as
- * gl_pbr_spmm_i32_f32). With 8x8 tiles, fused mode and 32 RHS columns per launch, the tiles
- * are multiplied on the matrix cores from the layout; every other case ignores it. */
+ * gl_pbr_spmm_i32_f32). With 8x8 tiles, fused mode and 8 or 32 RHS columns per launch, the
+ * tiles are multiplied on the matrix cores from the layout; every other case ignores it. */
 int gl_pbr_spmm_exec_i32_f32(int64_t rows, int64_t cols, int32_t block_rows,
                              int32_t block_cols, int64_t num_tiles, int64_t num_vals,
                              int64_t num_rem, const uint64_t* codes, const int32_t* coords,

@@ -83,12 +83,16 @@ int launch_stripe_bb(const PbrView& m, const RhsView& d, int kw, const Epi& e, i
   const bool full = std::min<int64_t>(kw, d.K - d.k0) == kw;
   const bool vec = BUF || ((d.ldx & 3) == 0 && full);
   if constexpr (B == 8 && BUF && TILES && std::is_same_v<Epi, EpiStore>) {
-    if (kw == 32 && m.trec != nullptr && m.tdata != nullptr) {
-      const bool aligned = (d.ldx & 3) == 0 && (d.k0 & 3) == 0 &&
-                           (reinterpret_cast<uintptr_t>(d.x) & 15u) == 0;
-      if (aligned) GL_GO((k_stripe_mx32<TILES, REM, true>));
-      else GL_GO((k_stripe_mx32<TILES, REM, false>));
-      if (!occ) GL_LAUNCH_CHECK("k_stripe_mx32");
+    if (m.trec != nullptr && m.tdata != nullptr && (kw == 32 || kw == 8)) {
+      if (kw == 8) {
+        GL_GO((k_stripe_mx<8, TILES, REM, false>));
+      } else if ((d.ldx & 3) == 0 && (d.k0 & 3) == 0 &&
+                 (reinterpret_cast<uintptr_t>(d.x) & 15u) == 0) {
+        GL_GO((k_stripe_mx<32, TILES, REM, true>));
+      } else {
+        GL_GO((k_stripe_mx<32, TILES, REM, false>));
+      }
+      if (!occ) GL_LAUNCH_CHECK("k_stripe_mx");
       return GL_OK;
     }
   }

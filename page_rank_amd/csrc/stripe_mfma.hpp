@@ -1,5 +1,7 @@
-// Stripe-owner SpMM kernel for 8x8 PBR matrices at 32 RHS columns whose tile products run on
-// the matrix cores (gfx950), reading the matrix's execution layout (TileRec below).
+// Stripe-owner SpMM kernel for 8x8 PBR matrices at 32 or 8 RHS columns per launch whose tile
+// products run on the matrix cores (gfx950), reading the matrix's execution layout (TileRec
+// below). The description is of the 32-column form; at 8 columns each lane holds one column
+// instead of four (one dword of the X block, two instructions per tile).
 //
 // It does the tile work of k_stripe_feat<8, 32> with fewer instructions: one dwordx4 X load
 // per tile (instead of four dword loads), tile metadata through scalar loads, a three-
@@ -103,14 +105,18 @@ __device__ __forceinline__ uint32_t sel_by_mask(uint32_t a, uint32_t b, uint64_t
   return r;
 }
 
-// ALIGNED: X rows and the launch's first column are 16-byte aligned, so each X block row
-// segment loads with one 128-bit buffer load whose row offset is a scalar.
-template <bool TILES, bool REM, bool ALIGNED>
+// KW: RHS columns of the launch, 32 (a float4 of columns per lane) or 8 (one column per
+// lane: the X block of a tile is one dword per lane, and each tile takes 2 instructions).
+// ALIGNED (32 columns): X rows and the launch's first column are 16-byte aligned, so each X
+// block row segment loads with one 128-bit buffer load whose row offset is a scalar.
+template <int KW, bool TILES, bool REM, bool ALIGNED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
-  constexpr int B = 8, KW = 32;
-  constexpr int L = 8;    // remainder: lanes per row
-  constexpr int FPL = 4;  // remainder: RHS columns per lane
+k_stripe_mx(PbrView m, RhsView d, EpiStore epi) {
+  static_assert(KW == 8 || KW == 32, "8 or 32 columns per launch");
+  constexpr int B = 8;
+  constexpr int L = 8;          // remainder: lanes per row
+  constexpr int CPL = KW / 8;   // RHS columns per lane (tiles and remainder)
+  constexpr int FPL = CPL;
   constexpr int EB = GL_REM_EB;
   __shared__ float vstage[4][kMxVals + 4];
 
@@ -120,13 +126,13 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int rl = lane >> 3;  // the lane's output row inside the stripe (and remainder row)
-  const int q = lane & 7;    // its quad of columns: 4q .. 4q + 3
+  const int q = lane & 7;    // its group of columns: CPL q .. CPL q + CPL - 1
   const int kvalid = (int)min<int64_t>(KW, d.K - d.k0);
-  const int nv = kvalid - 4 * q;  // columns of the quad inside the matrix
+  const int nv = kvalid - CPL * q;  // columns of the group inside the matrix
   const uint32_t ldb = (uint32_t)d.ldx * 4u;
   const bool b3 = (lane >> 3) & 1;
-  // byte offset of this lane's float4 of an X block (row c0 + (lane >> 3), quad q)
-  const uint32_t xo = ((uint32_t)(lane >> 3) * (uint32_t)d.ldx + 4u * (uint32_t)q) * 4u;
+  // byte offset of this lane's columns of an X block (row c0 + (lane >> 3), group q)
+  const uint32_t xo = ((uint32_t)(lane >> 3) * (uint32_t)d.ldx + (uint32_t)(CPL * q)) * 4u;
   float* __restrict__ vs = vstage[wid];
   const uint32_t zslot = 4u * kMxVals;  // byte address of the zero slot
   if (lane == 0) vs[kMxVals] = 0.f;
@@ -166,9 +172,11 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
       sval = ld.rval.f32(rs.j0 + q, ok);
     }
 
-    float out[4] = {0.f, 0.f, 0.f, 0.f};
+    float out[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) out[j] = 0.f;
     if constexpr (TILES) {
-      v4f acc[2][4];
+      v4f acc[2][CPL];
       // One batch of up to 8 tiles from tile tb. FIRST: the stripe's first batch, whose first
       // tile starts the accumulators (a zero C operand instead of zeroing them).
       auto batch = [&](int tb, auto first_tag) {
@@ -193,13 +201,16 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
         const float4 v0 = tval.f32x4(boff + 4 * lane, 4 * lane < nval);
         float4 v1 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (nval > 256) v1 = tval.f32x4(boff + 256 + 4 * lane, 256 + 4 * lane < nval);
-        float xv[kMxBatch][4];
+        float xv[kMxBatch][CPL];
 #pragma unroll
         for (int u = 0; u < kMxBatch; ++u) {
           // a slot past the batch's last tile reads that tile's X block (its mask is empty)
           const int c0 = u < nb ? r[u].c0 : r[0].c0;
           const uint32_t so = (uint32_t)c0 * ldb;
-          if constexpr (ALIGNED) {
+          if constexpr (CPL == 1) {
+            xv[u][0] = __builtin_bit_cast(
+                float, __builtin_amdgcn_raw_buffer_load_b32(ld.x.r, xo, so, 0));
+          } else if constexpr (ALIGNED) {
             const v4f t = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(ld.x.r, xo, so, 0));
             xv[u][0] = t.x;
             xv[u][1] = t.y;
@@ -232,7 +243,7 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
 #pragma unroll
         for (int u = 0; u < kMxBatch; ++u) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
+          for (int j = 0; j < CPL; ++j) {
             const v4f c0v = (FIRST && u == 0) ? v4f{0.f, 0.f, 0.f, 0.f} : acc[0][j];
             const v4f c1v = (FIRST && u == 0) ? v4f{0.f, 0.f, 0.f, 0.f} : acc[1][j];
             acc[0][j] = __builtin_amdgcn_mfma_f32_4x4x1f32(av[u], xv[u][j], c0v, 1, 0, 0);
@@ -245,7 +256,7 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
         for (int tb = ta0 + kMxBatch; tb < ta1; tb += kMxBatch) batch(tb, std::false_type{});
         // add the 8 tile columns' partial sums (lane bits 5, 4, 3 <-> row bits h, v1, v0)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CPL; ++j) {
           float r1[4];
 #pragma unroll
           for (int v = 0; v < 4; ++v) r1[v] = swap32_sum(acc[0][j][v], acc[1][j][v]);
@@ -277,7 +288,7 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
             const int c = __shfl(scol, gbase + ib + i, kWave);
             a[i] = __shfl(sval, gbase + ib + i, kWave);
             const bool ok = jb + ib + i < rs.n();
-            load_row<FPL>(ld.xr, xrow<true>(c, d.ldx) + 4 * q, true, ok ? nv : 0, xr[i]);
+            load_row<FPL>(ld.xr, xrow<true>(c, d.ldx) + CPL * q, true, ok ? nv : 0, xr[i]);
           }
 #pragma unroll
           for (int i = 0; i < EB; ++i)
@@ -291,8 +302,11 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
 
     const int64_t row = r0 + rl;
     if (row < m.rows && nv > 0) {
-      float* __restrict__ yp = epi.y + elem(row, epi.ldy, d.k0 + 4 * q, epi.u24);
-      if (nv >= 4 && (reinterpret_cast<uintptr_t>(yp) & 15u) == 0) {
+      float* __restrict__ yp = epi.y + elem(row, epi.ldy, d.k0 + CPL * q, epi.u24);
+      if constexpr (CPL == 1) {
+        if (epi.accumulate) *yp += out[0];
+        else __builtin_nontemporal_store(out[0], yp);
+      } else if (nv >= 4 && (reinterpret_cast<uintptr_t>(yp) & 15u) == 0) {
         float4 o = make_float4(out[0], out[1], out[2], out[3]);
         if (epi.accumulate) {
           const float4 y0 = *reinterpret_cast<const float4*>(yp);
@@ -306,7 +320,7 @@ k_stripe_mx32(PbrView m, RhsView d, EpiStore epi) {
         __builtin_nontemporal_store(v4f{o.x, o.y, o.z, o.w}, reinterpret_cast<v4f*>(yp));
       } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CPL; ++j)
           if (j < nv) yp[j] = epi.accumulate ? yp[j] + out[j] : out[j];
       }
     }

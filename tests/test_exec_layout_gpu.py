@@ -1,5 +1,5 @@
 """GPU checks of the 8x8 execution layout (gl_pbr_exec_layout) and of the matrix-core tile
-kernel that reads it (32 RHS columns per launch, fused mode).
+kernel that reads it (8 and 32 RHS columns per launch, fused mode).
 
 The layout is checked against its definition computed here with numpy (transposed codes,
 column-major values, zero padding records); the product against scipy in FP64 on matrices
@@ -88,38 +88,43 @@ def _check(A, X, Y, Y0=None):
     assert np.max(err / (scale + (np.abs(Y0) if Y0 is not None else 0))) < 1e-5, np.max(err)
 
 
-@pytest.mark.parametrize("n,K,ldx_pad,seed", [
-    (203, 32, 0, 2),     # last block row and column past the edge, aligned rows
-    (256, 32, 1, 3),     # row pitch 33: the dword-load form
-    (256, 45, 0, 4),     # a 32-column launch, then a partial chunk in another kernel
-    (520, 64, 0, 5),     # two 32-column launches
-    (640, 32, 3, 6),
+@pytest.mark.parametrize("n,K,kw,ldx_pad,seed", [
+    (203, 32, 32, 0, 2),     # last block row and column past the edge, aligned rows
+    (256, 32, 32, 1, 3),     # row pitch 33: the dword-load form
+    (256, 45, 32, 0, 4),     # a 32-column launch, then a partial chunk in another kernel
+    (520, 64, 32, 0, 5),     # two 32-column launches
+    (640, 32, 32, 3, 6),
+    (203, 8, 8, 0, 7),       # 8 columns per launch
+    (256, 5, 8, 2, 8),       # a partial 8-column launch, unaligned pitch
+    (520, 24, 8, 1, 9),      # three 8-column launches
 ])
-def test_product_matches_fp64(n, K, ldx_pad, seed, device):
+def test_product_matches_fp64(n, K, kw, ldx_pad, seed, device):
     A = _matrix(n, seed, dense_stripes=3, wide_stripe=True)
     pbr = gl.csr_to_pbr(A, 8, 8, 4).to(device)
     rng = np.random.default_rng(seed)
     Xfull = rng.random((n, K + ldx_pad), dtype=np.float32)
     X = torch.from_numpy(Xfull).to(device)[:, :K]
     Y = torch.empty((n, K), dtype=torch.float32, device=device)
-    gl.spmm_into(pbr, X, Y, kw=32)
+    gl.spmm_into(pbr, X, Y, kw=kw)
     _check(A, Xfull[:, :K], Y.cpu().numpy())
     assert "tile_records" in gl.get_pbr_gpu_meta(pbr)
     # accumulate into Y
     Y0 = rng.random((n, K), dtype=np.float32)
     Y = torch.from_numpy(Y0.copy()).to(device)
-    gl.spmm_into(pbr, X, Y, kw=32, accumulate=True)
+    gl.spmm_into(pbr, X, Y, kw=kw, accumulate=True)
     _check(A, Xfull[:, :K], Y.cpu().numpy(), Y0.astype(np.float64))
 
 
-def test_exec_path_equals_layout_free_path(device):
-    """The matrix-core kernel against the lane-per-column kernel on the same product."""
+@pytest.mark.parametrize("K", [32, 8])
+def test_exec_path_equals_layout_free_path(K, device):
+    """The matrix-core kernel against the lane-per-column / bit-lane kernel on the same
+    product."""
     A = _matrix(1024, 7, dense_stripes=4, wide_stripe=True, density=0.02)
     pbr = gl.csr_to_pbr(A, 8, 8, 4).to(device)
-    X = torch.rand((1024, 32), dtype=torch.float32, device=device)
-    Y1 = torch.empty((1024, 32), dtype=torch.float32, device=device)
+    X = torch.rand((1024, K), dtype=torch.float32, device=device)
+    Y1 = torch.empty((1024, K), dtype=torch.float32, device=device)
     Y2 = torch.empty_like(Y1)
-    gl.spmm_into(pbr, X, Y1, kw=32)                 # execution layout
+    gl.spmm_into(pbr, X, Y1, kw=K)                  # execution layout
     meta = gl.get_pbr_gpu_meta(pbr)
     L = _native.lib()
     rc = L.gl_pbr_spmm_i32_f32(1024, 1024, 8, 8, meta["codes"].numel(), meta["data"].numel(),
@@ -127,13 +132,13 @@ def test_exec_path_equals_layout_free_path(device):
                                meta["coords"].data_ptr(), meta["offsets"].data_ptr(),
                                meta["data"].data_ptr(), meta["stripe_ptr"].data_ptr(),
                                meta["rem_indptr"].data_ptr(), meta["rem_col_ind"].data_ptr(),
-                               meta["rem_vals"].data_ptr(), X.data_ptr(), 32, 32, Y2.data_ptr(),
-                               32, 0, 0, 32, _native.stream_handle(device))
+                               meta["rem_vals"].data_ptr(), X.data_ptr(), K, K, Y2.data_ptr(),
+                               K, 0, 0, K, _native.stream_handle(device))
     _native.check(rc, "gl_pbr_spmm_i32_f32")
     d = (Y1 - Y2).abs().max().item()
     scale = Y2.abs().max().item()
     assert d <= 1e-6 * scale, d
     # two runs of the matrix-core kernel agree bit for bit
     Y3 = torch.empty_like(Y1)
-    gl.spmm_into(pbr, X, Y3, kw=32)
+    gl.spmm_into(pbr, X, Y3, kw=K)
     assert torch.equal(Y1, Y3)
