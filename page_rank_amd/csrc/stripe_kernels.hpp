@@ -26,11 +26,10 @@
 #ifndef GL_REM_EB
 #define GL_REM_EB 4
 #endif
-// The same for the 64-column feature-lane kernel. Measured: 1 entry keeps the PPR step at
-// 74 VGPRs (6 waves per SIMD) and is 1 % faster than 2 there; 2 was 5 % faster than 4 with
-// the registers of an earlier form of the kernel; 4 is 12 % faster at 32 columns.
+// The same for the 64-column feature-lane kernel. Measured on the 4M-node PPR step, with the
+// loop bound a scalar: 2 entries 883 PPR/s, 1 entry 857, 4 entries 844 (fewer waves).
 #ifndef GL_REM_EB64
-#define GL_REM_EB64 1
+#define GL_REM_EB64 2
 #endif
 // The same for the plain-pointer form (arrays of 2 GB or more): 1 entry measured 15 % slower
 // than 2 for the 16M-node PPR step.
@@ -429,10 +428,12 @@ __device__ __forceinline__ int64_t xrow(int64_t r, int64_t ldx) {
   if constexpr (BUF) return (int64_t)__umul24((uint32_t)r, (uint32_t)ldx);
   else return r * ldx;
 }
+// The result is returned as a scalar: loops bounded by it are then uniform, which lets the
+// compiler keep their loads in flight together instead of masking every iteration.
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int m = 1; m < kWave; m <<= 1) v = max(v, __shfl_xor(v, m, kWave));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Interleaved form of load_row for a lane slice of N = 4 H floats (H >= 2) among L lanes of

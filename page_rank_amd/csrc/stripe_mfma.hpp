@@ -94,7 +94,7 @@ __device__ __forceinline__ int wave_max_fast(int v) {
   const auto a = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
   v = max((int)a[0], (int)a[1]);
   const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
-  return max((int)b[0], (int)b[1]);
+  return __builtin_amdgcn_readfirstlane(max((int)b[0], (int)b[1]));
 }
 
 // v_cndmask with a wave-uniform 64-bit lane mask: lane l gets b if bit l of mask is set,
