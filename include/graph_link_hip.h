@@ -189,28 +189,33 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
  * Backend/graph_link/graph_link/__init__.py:137-156). Input: canonical CSR on the device
  * (int32 indptr/indices, rows sorted by column, no duplicates); values are copied as opaque
  * 4- or 8-byte payloads, so float32 and float64 matrices build bit-exactly.
- * Two calls on the same workspace:
+ * Two calls on the same workspace (gl_pbr_build_workspace_bytes(rows, block_rows, nnz):
+ * per-stripe counts plus 14 bytes per CSR entry, which carry each entry's block and bit from
+ * the count pass to the fill pass so that the fill pass neither reads column indices nor
+ * hashes):
  *   gl_pbr_build_count: fills rem_indptr[rows+1] and returns counts[4] = {T, V, R, D}
  *                       (tiles, tile values, remainder entries, distinct blocks) on the host;
  *   gl_pbr_build_fill : writes codes[T], coords[2T], offsets[T+1], data[V], rem_col_ind[R],
  *                       rem_vals[R] in canonical order (block columns ascending inside a
- *                       stripe) and, when blk_bc != NULL, the distinct-block list
+ *                       stripe), stripe_ptr[ns+1] (first tile of each stripe) when not NULL,
+ *                       and, when blk_bc != NULL, the distinct-block list
  *                       blk_ptr[ns+1], blk_bc[D], blk_rmin[D], blk_tile[D] used to restore
  *                       the reference's emission order.
  * Block sizes: any block_rows x block_cols <= 64 (the reference's bitset<64> code).
  * ---------------------------------------------------------------------------------- */
-int64_t gl_pbr_build_workspace_bytes(int64_t rows, int32_t block_rows);
-int gl_pbr_build_count(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                       int32_t min_nnz_per_block, const int32_t* indptr,
+int64_t gl_pbr_build_workspace_bytes(int64_t rows, int32_t block_rows, int64_t nnz);
+int gl_pbr_build_count(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
+                       int32_t block_cols, int32_t min_nnz_per_block, const int32_t* indptr,
                        const int32_t* indices, int32_t* rem_indptr, void* workspace,
                        int64_t workspace_bytes, int64_t* counts, void* stream);
-int gl_pbr_build_fill(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                      int32_t min_nnz_per_block, int64_t num_tiles, const int32_t* indptr,
-                      const int32_t* indices, const void* values, int32_t value_bytes,
-                      void* workspace, int64_t workspace_bytes, uint64_t* codes,
-                      int32_t* coords, int32_t* offsets, void* data, const int32_t* rem_indptr,
-                      int32_t* rem_col_ind, void* rem_vals, int32_t* blk_ptr, int32_t* blk_bc,
-                      int8_t* blk_rmin, int32_t* blk_tile, void* stream);
+int gl_pbr_build_fill(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
+                      int32_t block_cols, int32_t min_nnz_per_block, int64_t num_tiles,
+                      const int32_t* indptr, const int32_t* indices, const void* values,
+                      int32_t value_bytes, void* workspace, int64_t workspace_bytes,
+                      uint64_t* codes, int32_t* coords, int32_t* offsets, void* data,
+                      const int32_t* rem_indptr, int32_t* rem_col_ind, void* rem_vals,
+                      int32_t* stripe_ptr, int32_t* blk_ptr, int32_t* blk_bc, int8_t* blk_rmin,
+                      int32_t* blk_tile, void* stream);
 /* Reference emission order: replays the reference builder's std::unordered_map<index_t,...>
  * (bucket hint sqrt(num_col_blocks), inserts in CSR first-occurrence order, clear() per
  * stripe; pbr_matrix.cpp:374-428) on HOST copies of the block list and returns perm[T]

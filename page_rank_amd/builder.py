@@ -13,7 +13,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _native
-from .pbr import pbr_class, stripe_index
+from .pbr import pbr_class
 
 
 def _as_csr_arrays(mat):
@@ -84,11 +84,11 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
     vb = 8 if st == np.float64 else 4
     with torch.cuda.device(dev):
         stream = _native.stream_handle(dev)
-        wsb = L.gl_pbr_build_workspace_bytes(rows, block_rows)
+        wsb = L.gl_pbr_build_workspace_bytes(rows, block_rows, nnz)
         ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
         rem_indptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
         counts = np.zeros(4, dtype=np.int64)
-        _native.check(L.gl_pbr_build_count(rows, cols, block_rows, block_cols,
+        _native.check(L.gl_pbr_build_count(rows, cols, nnz, block_rows, block_cols,
                                            int(min_nnz_per_block), indptr.data_ptr(),
                                            indices.data_ptr(), rem_indptr.data_ptr(),
                                            ws.data_ptr(), wsb, counts.ctypes.data, stream),
@@ -102,6 +102,7 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
         data = torch.empty(V, dtype=vdt, device=dev)
         rem_col = torch.empty(R, dtype=torch.int32, device=dev)
         rem_vals = torch.empty(R, dtype=vdt, device=dev)
+        stripe_ptr = torch.empty(ns + 1, dtype=torch.int32, device=dev)
         want_blocks = order == "reference" and T > 1
         if want_blocks:
             blk_ptr = torch.empty(ns + 1, dtype=torch.int32, device=dev)
@@ -112,10 +113,11 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
             blk_ptr = blk_bc = blk_rmin = blk_tile = None
         p = _native.ptr
         _native.check(L.gl_pbr_build_fill(
-            rows, cols, block_rows, block_cols, int(min_nnz_per_block), T, indptr.data_ptr(),
-            indices.data_ptr(), vals.data_ptr(), vb, ws.data_ptr(), wsb, p(codes), p(coords),
-            offsets.data_ptr(), p(data), rem_indptr.data_ptr(), p(rem_col), p(rem_vals),
-            p(blk_ptr), p(blk_bc), p(blk_rmin), p(blk_tile), stream), "gl_pbr_build_fill")
+            rows, cols, nnz, block_rows, block_cols, int(min_nnz_per_block), T,
+            indptr.data_ptr(), indices.data_ptr(), vals.data_ptr(), vb, ws.data_ptr(), wsb,
+            p(codes), p(coords), offsets.data_ptr(), p(data), rem_indptr.data_ptr(),
+            p(rem_col), p(rem_vals), stripe_ptr.data_ptr(), p(blk_ptr), p(blk_bc), p(blk_rmin),
+            p(blk_tile), stream), "gl_pbr_build_fill")
         if want_blocks:
             h_ptr = blk_ptr.cpu().numpy()
             h_bc = blk_bc.cpu().numpy()
@@ -145,5 +147,5 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
             codes, coords, offsets, data = codes2, coords2, offsets2, data2
         meta = {"codes": codes, "coords": coords, "offsets": offsets, "data": data,
                 "rem_indptr": rem_indptr, "rem_col_ind": rem_col, "rem_vals": rem_vals}
-        meta["stripe_ptr"] = stripe_index(coords, rows, block_rows, T, dev)
+        meta["stripe_ptr"] = stripe_ptr  # tiles stay in their stripe in either order
     return cls(rows, cols, block_rows, block_cols, nnz, device_meta=meta)

@@ -17,6 +17,7 @@
 // replay the reference's container operations and gl_pbr_permute_tiles can reorder.
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <cmath>
 #include <unordered_map>
 #include <vector>
@@ -75,6 +76,11 @@ struct FillOut {
   int32_t* blk_tile;
   uint8_t* big;       // stripes left to the merge kernel (more distinct blocks than the
                       // LDS table holds); NULL = the merge kernel takes every stripe
+  int32_t* stripe_ptr;  // optional: first tile of each stripe [ns+1]
+  // count pass -> fill pass, indexed by CSR entry
+  uint64_t* wcode;    // block codes of a stripe's distinct blocks, from its first entry on
+  int32_t* wkey;      // their block columns
+  uint16_t* winfo;    // per entry: block index | bit << 8
 };
 
 template <int GL, bool FILL, typename V>
@@ -187,43 +193,50 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Wave-per-stripe builder. A stripe's entries are one contiguous CSR segment; the wave reads
-// it 64 entries at a time (coalesced) and inserts every entry's block column into an LDS
-// hash table (atomic CAS on the key, atomic OR of the entry's bit into the code, atomic min
-// of its row). The occupied keys are then put in ascending order (rank = number of smaller
-// keys, counted with broadcast LDS reads), which gives
-// the canonical tile order, tile indices and value offsets by prefix sums; a second sweep
-// over the entries writes kept values at popcount(code & bits below) and rejected entries
-// at their rank inside their row (ballot prefix counts), so remainder rows keep column
-// order. Results are the merge kernel's, entry for entry. Stripes with more than kHashCap
-// distinct blocks are flagged and left to the merge kernel.
+// Wave-per-stripe builder, two passes that hash each entry once.
+//
+// Count pass: a wave reads its stripe's CSR segment 64 entries at a time (coalesced) and
+// inserts every entry's block column into an LDS hash table (compare-and-swap on the key,
+// 64-bit OR of the entry's bit into the code). It then lists the occupied blocks (in table
+// order, j = 0..D-1) into workspace arrays indexed by the stripe's first entry E0 (a stripe
+// never has more distinct blocks than entries, so the list fits in [E0, E0 + D)), writes
+// every entry's block index and bit as one 16-bit word, and counts tiles and tile values per
+// stripe and remainder entries per row (a row's entries are a lane range of each chunk).
+//
+// Fill pass (after the scans): the wave reads the stripe's block list back, ranks the kept
+// blocks by block column (canonical tile order), gives each its tile index and value offset
+// by prefix sums, and places every entry from its 16-bit word alone: a kept entry at
+// value_offset + popcount(code & bits below its bit), a rejected one at the stripe's
+// remainder offset + its rank among the stripe's rejected entries. Remainder rows are stored
+// one after the other and each keeps CSR (column) order, so that rank is the entry's
+// position in its remainder row plus the remainder entries of the rows above it; its column
+// is block * block_cols + bit % block_cols. The fill pass reads no column index and does no
+// hashing. Stripes with more than kHashCap distinct blocks are flagged by the count pass and
+// left to the merge kernel in both passes. Results equal the merge kernel's entry for entry.
 // ---------------------------------------------------------------------------------------
 constexpr int kHashCap = 256;
-// Register budget of the fill pass: 7 waves per SIMD (72 VGPRs), which is also what its LDS
-// allows (7 workgroups of 4 waves per CU).
-constexpr int kHashWavesPerSimd = 7;
-// Grid of the count pass, in workgroups per CU: 7 of the 8 its LDS allows (0.71 ms for the
-// whole 1M x 1M build, against 0.77 ms with 8 and with 4).
-constexpr int kHashCountWgPerCu = 7;
-// Chunks of 64 entries per stripe whose column (and, in the fill pass, value) loads leave
-// together at the top of the stripe and stay in registers for both sweeps: one memory round
-// trip instead of one per chunk per sweep. Longer stripes load the rest chunk by chunk.
+// Grid of the count pass, in workgroups per CU (its LDS allows 9).
+constexpr int kHashCountWgPerCu = 8;
+// Chunks of 64 entries per stripe whose loads leave together at the top of the stripe and
+// stay in registers for the stripe's sweeps. Longer stripes load the rest chunk by chunk.
 constexpr int kHashPre = 4;
+// stripes of at most kHashCap entries are loaded whole by the prefetched chunks
+static_assert(kHashPre * kWave == kHashCap, "prefetched chunks must cover kHashCap entries");
 
-// FILL: the fill pass's arrays (value offsets, slots in key order) exist only there, so the
-// count pass needs less LDS per wave and holds more waves. A block's first row is not kept:
-// it is the lowest set bit of its code divided by block_cols.
-template <bool FILL>
-struct HashLds {
-  static constexpr int kF = FILL ? kHashCap : 1;
+struct CountLds {
+  unsigned long long code[kHashCap];
   int32_t key[kHashCap];
-  uint32_t lo[kHashCap], hi[kHashCap];
-  int32_t voff[kHashCap];  // per slot when kept: offset of its first value (fill) or 1 (count);
-                           // -1 if not
-  int32_t sslot[kF];       // slots in ascending key order
-  int32_t rows[65];        // CSR start of each row of the stripe (block_rows + 1 values)
-  int32_t rcnt[64];        // rejected entries per row so far
-  int32_t flag;            // table overflow
+  int32_t jof[kHashCap];  // slot -> list index j, | 256 when the block is rejected
+  int32_t rows[65];       // CSR start of each row of the stripe (block_rows + 1 values)
+  int32_t flag;           // table overflow
+};
+
+struct FillLds {
+  unsigned long long code[kHashCap];  // block j's code
+  int32_t key[kHashCap];              // block j's block column
+  int32_t voff[kHashCap];             // block j's first value offset when kept, else -1
+  int32_t lkey[kHashCap];             // listed blocks: block columns
+  uint8_t lidx[kHashCap];             // listed blocks: j, then j in ascending block column
 };
 
 __device__ __forceinline__ void wsync() {
@@ -231,8 +244,14 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// kHashCap = 256 slots: bits 16..23 of the low 24 bits of b times a Fibonacci-hashing
+// constant (a full-rate 24-bit multiply)
 __device__ __forceinline__ int hash_slot(int32_t b) {
-  return (int)(((uint32_t)b * 2654435761u) >> 24);  // kHashCap = 256
+  return (int)((__umul24((uint32_t)b, 0x9E3779u) >> 16) & 255u);
+}
+// bits [0, n) of a 64-bit mask, n in [0, 64]
+__device__ __forceinline__ uint64_t low_bits(int n) {
+  return n >= 64 ? ~0ull : ((1ull << (uint64_t)n) - 1ull);
 }
 // Row (inside the stripe) of entry e: the last k < br with rows[k] <= e.
 __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
@@ -246,82 +265,79 @@ __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
 }
 
 // P2: block_cols is a power of two, so an entry's block column and bit come from a shift and
-// a mask instead of an integer division (the kernel is bound by VALU issue).
-template <bool FILL, typename V, bool P2>
-__global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
-    k_build_hash(BuildArgs a, FillOut o) {
-  __shared__ HashLds<FILL> sh_all[4];
+// a mask instead of an integer division (both passes are bound by VALU issue).
+template <bool P2>
+__global__ void __launch_bounds__(256) k_hash_count(BuildArgs a, FillOut o) {
+  __shared__ CountLds sh_all[4];
   const int lane = threadIdx.x & (kWave - 1);
-  HashLds<FILL>& h = sh_all[threadIdx.x / kWave];
+  CountLds& h = sh_all[threadIdx.x / kWave];
   // uniform across the wave: scalar registers
   const int64_t wave = __builtin_amdgcn_readfirstlane(
       (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
-  const V* __restrict__ vals = (const V*)o.values;
   const int br = a.br, bc = a.bc;
   const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
-  const uint64_t below_lane = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  // block column of a column, and the entry's bit in its block's code (row r of the stripe)
+  const uint64_t below_lane = low_bits(lane);
   auto block_of = [&](int32_t col) { return P2 ? (col >> bc_shift) : col / bc; };
   auto bit_of = [&](int r, int32_t col, int32_t b) {
     return P2 ? ((r << bc_shift) | (col & (bc - 1))) : r * bc + (col - b * bc);
   };
-
-  // Per-stripe loads that do not depend on the stripe's entries: CSR row starts (row k of the
-  // stripe in lane k, row 64 in lane 0's second register) and, in the fill pass, the stripe's
-  // output offsets and its rows' remainder offsets. They are issued a stripe ahead.
-  struct StripeHead {
-    int32_t rp0, rp1, nr;
-    int32_t nt, nv, nd;
-  };
-  auto load_head = [&](int64_t st) {
-    StripeHead hd{0, 0, 0, 0, 0, 0};
-    if (st >= a.num_stripes) return hd;
+  // CSR row starts of a stripe: row k in lane k, row 64 in lane 0's second register
+  auto load_rows = [&](int64_t st, int32_t& rp0, int32_t& rp1) {
+    rp0 = rp1 = 0;
+    if (st >= a.num_stripes) return;
     const int64_t rb = st * br;
-    if (lane <= br) hd.rp0 = a.indptr[min<int64_t>(rb + lane, a.rows)];
-    if (lane == 0 && br >= kWave) hd.rp1 = a.indptr[min<int64_t>(rb + kWave, a.rows)];
-    if constexpr (FILL) {
-      if (lane < br && rb + lane < a.rows) hd.nr = o.nr[rb + lane];
-      hd.nt = o.nt[st];
-      hd.nv = o.nv[st];
-      hd.nd = o.nd[st];
-    }
-    return hd;
+    if (lane <= br) rp0 = a.indptr[min<int64_t>(rb + lane, a.rows)];
+    if (lane == 0 && br >= kWave) rp1 = a.indptr[min<int64_t>(rb + kWave, a.rows)];
   };
-  StripeHead head = load_head(wave);
+  auto end_of = [&](int32_t rp0, int32_t rp1) {
+    return br >= kWave ? __builtin_amdgcn_readfirstlane(rp1) : __builtin_amdgcn_readlane(rp0, br);
+  };
+  auto load_cols = [&](int32_t e0, int32_t e1, int32_t* col) {
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t e = e0 + k * kWave + lane;
+      col[k] = e < e1 ? a.indices[e] : 0;
+    }
+  };
+  // Software pipeline: a stripe's row starts are loaded two stripes ahead and its first
+  // column chunks one stripe ahead, both before the previous stripe's stores, so the wait
+  // for a stripe's columns overlaps the previous stripe's stores instead of following them.
+  int32_t rp0, rp1, nrp0, nrp1;
+  load_rows(wave, rp0, rp1);
+  int32_t E0 = __builtin_amdgcn_readfirstlane(rp0), E1 = end_of(rp0, rp1);
+  int32_t pcol[kHashPre];
+  load_cols(E0, E1, pcol);
+  load_rows(wave + nwaves, nrp0, nrp1);
 
   for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
     const int64_t r0 = s * br;
-    const StripeHead cur = head;
-    if (lane <= br) h.rows[lane] = cur.rp0;
-    if (lane == 0 && br >= kWave) h.rows[kWave] = cur.rp1;
-    // fill: remainder entries of each row so far start at the row's remainder offset
-    if (lane < br) h.rcnt[lane] = FILL ? cur.nr : 0;
+    if (lane <= br) h.rows[lane] = rp0;
+    if (lane == 0 && br >= kWave) h.rows[kWave] = rp1;
     for (int i = lane; i < kHashCap; i += kWave) {
       h.key[i] = -1;
-      h.lo[i] = 0u;
-      h.hi[i] = 0u;
+      h.code[i] = 0ull;
     }
     if (lane == 0) h.flag = 0;
-    wsync();
-    const int32_t E0 = __builtin_amdgcn_readfirstlane(h.rows[0]);
-    const int32_t E1 = __builtin_amdgcn_readfirstlane(h.rows[br]);
-    // this stripe's first kHashPre chunks, then the next stripe's head
-    int32_t pcol[kHashPre];
-    V pval[kHashPre];
-#pragma unroll
-    for (int k = 0; k < kHashPre; ++k) {
-      const int32_t e = E0 + k * kWave + lane;
-      pcol[k] = e < E1 ? a.indices[e] : 0;
-      if constexpr (FILL) pval[k] = e < E1 ? vals[e] : V(0);
-    }
-    head = load_head(s + nwaves);
-    // row starts 1..7 in registers for the common block_rows <= 8 (a row is then found by
+    // row starts 1..7 as scalars for the common block_rows <= 8 (a row is then found by
     // compares instead of a search in LDS)
     int32_t rs8[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) rs8[k] = k + 1 < br ? h.rows[k + 1] : INT32_MAX;
-    auto row_at = [&](int32_t e) {
+    for (int k = 0; k < 7; ++k)
+      rs8[k] = k + 1 < br ? __builtin_amdgcn_readlane(rp0, k + 1) : INT32_MAX;
+    // lane k < br: row k covers entries [rk, rk1)
+    // (the 65th row start sits in lane 0's second register; read it with the whole wave
+    // active)
+    const int32_t rk = rp0;
+    const int32_t row64 = __builtin_amdgcn_readfirstlane(rp1);
+    const int32_t next_start = __shfl_down(rp0, 1, kWave);
+    const int32_t rk1 = (br >= kWave && lane == kWave - 1) ? row64 : next_start;
+    wsync();
+    // Row of each lane's entry in the chunk of 64 entries from c: compares against the
+    // scalar row starts for block_rows <= 8 (a scalar mask of row-start lanes counted with
+    // mbcnt took more scalar instructions than it saved), a search in LDS for taller blocks.
+    auto chunk_row = [&](int32_t c) {
+      const int32_t e = c + lane;
       if (br <= 8) {
         int r = 0;
 #pragma unroll
@@ -330,12 +346,11 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
       }
       return row_of(h.rows, br, e);
     };
-
-    // ---- insert every entry's block
-    // returns the entry's slot | bit << 8 | row << 14 (-1 if the table is full), which the
-    // entry sweep takes instead of finding the slot again
-    auto insert = [&](int32_t e, int32_t col) -> int32_t {
-      const int r = row_at(e);
+    // Returns the entry's slot | bit << 8. CHECK: the table may fill (more than kHashCap
+    // entries in the stripe); -1 and the overflow flag then. With at most kHashCap entries a
+    // full table holds every key of the stripe, so the probe always ends.
+    auto insert = [&](auto check_tag, int r, int32_t col) -> int32_t {
+      constexpr bool CHECK = decltype(check_tag)::value;
       const int32_t b = block_of(col);
       const int bit = bit_of(r, col, b);
       int slot = hash_slot(b);
@@ -344,124 +359,269 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         const int32_t old = atomicCAS(&h.key[slot], -1, b);
         if (old == -1 || old == b) break;
         slot = (slot + 1) & (kHashCap - 1);
-        if (++probes == kHashCap) {
+        if (CHECK && ++probes == kHashCap) {
           slot = -1;
           break;
         }
       }
-      if (slot < 0) {
+      if (CHECK && slot < 0) {
         h.flag = 1;
         return -1;
       }
-      if (bit < 32) atomicOr(&h.lo[slot], 1u << bit);
-      else atomicOr(&h.hi[slot], 1u << (bit - 32));
-      return slot | (bit << 8) | (r << 14);
+      atomicOr(&h.code[slot], 1ull << (uint64_t)bit);
+      return slot | (bit << 8);
     };
     int32_t pinf[kHashPre];
+    if (E1 - E0 <= kHashCap) {
 #pragma unroll
-    for (int k = 0; k < kHashPre; ++k) {
-      const int32_t e = E0 + k * kWave + lane;
-      pinf[k] = e < E1 ? insert(e, pcol[k]) : -1;
+      for (int k = 0; k < kHashPre; ++k) {
+        const int32_t c = E0 + k * kWave;
+        const int r = chunk_row(c);
+        pinf[k] = c + lane < E1 ? insert(std::false_type{}, r, pcol[k]) : -1;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kHashPre; ++k) {
+        const int32_t c = E0 + k * kWave;
+        const int r = chunk_row(c);
+        pinf[k] = c + lane < E1 ? insert(std::true_type{}, r, pcol[k]) : -1;
+      }
+      for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
+        const int32_t e = c + lane;
+        const int r = chunk_row(c);
+        if (e < E1) insert(std::true_type{}, r, a.indices[e]);
+      }
     }
-    for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
-      const int32_t e = c + lane;
-      if (e < E1) insert(e, a.indices[e]);
+    // ---- next stripe: its columns and the row starts of the one after, issued before this
+    // stripe's stores
+    const int32_t cE0 = E0, cE1 = E1;
+    rp0 = nrp0;
+    rp1 = nrp1;
+    if (s + nwaves < a.num_stripes) {
+      E0 = __builtin_amdgcn_readfirstlane(rp0);
+      E1 = end_of(rp0, rp1);
+      load_cols(E0, E1, pcol);
+      load_rows(s + 2 * nwaves, nrp0, nrp1);
     }
     wsync();
     if (h.flag) {
-      if (!FILL && lane == 0) o.big[s] = 1;  // the merge kernel takes this stripe
+      if (lane == 0) o.big[s] = 1;  // the merge kernel takes this stripe
       wsync();
       continue;
     }
+    if (lane == 0) o.big[s] = 0;
 
-    if constexpr (!FILL) {
-      // counts do not depend on the tile order: sweep the table as it is (per-row remainder
-      // counts from the kept codes, with LDS atomics, measured slower than the entry sweep)
-      int32_t nd = 0, nt = 0, nv = 0;
-#pragma unroll
-      for (int t = 0; t < kHashCap / kWave; ++t) {
-        const int i = t * kWave + lane;
-        const bool occ = h.key[i] >= 0;
-        const int32_t cnt = __popc(h.lo[i]) + __popc(h.hi[i]);
-        const bool kept = occ && cnt >= a.min_nnz;
-        h.voff[i] = kept ? 1 : -1;
-        nd += occ ? 1 : 0;
-        nt += kept ? 1 : 0;
-        nv += kept ? cnt : 0;
-      }
-#pragma unroll
-      for (int m = 1; m < kWave; m <<= 1) {
-        nd += __shfl_xor(nd, m, kWave);
-        nt += __shfl_xor(nt, m, kWave);
-        nv += __shfl_xor(nv, m, kWave);
-      }
-      if (lane == 0) {
-        o.nt[s] = nt;
-        o.nv[s] = nv;
-        o.nd[s] = nd;
-      }
-    } else {
-    // ---- listed keys, compacted (keys into voff[], free until the kept blocks are numbered,
-    // slots into sslot[]), then ranked: sslot[] lists the slots in ascending key order. The
-    // list holds every occupied block when the reference-order block list is written, else
-    // only the kept ones (on remainder-heavy graphs a stripe has a few kept blocks among
-    // a hundred), whose voff[] entries are the only ones set >= 0 below.
-    const bool all_blocks = o.blk_bc != nullptr;
-    int32_t D = 0;
+    // ---- list the occupied blocks (table order) and count kept tiles and their values
+    int32_t D = 0, nt = 0, nv = 0;
 #pragma unroll
     for (int t = 0; t < kHashCap / kWave; ++t) {
       const int i = t * kWave + lane;
       const int32_t k = h.key[i];
-      const bool listed =
-          k >= 0 && (all_blocks || __popc(h.lo[i]) + __popc(h.hi[i]) >= a.min_nnz);
-      const uint64_t m = __ballot(listed);
-      if (listed) {
-        const int pos = D + __popcll(m & below_lane);
-        h.voff[pos] = k;
-        h.sslot[pos] = i;
+      const bool occ = k >= 0;
+      const uint64_t code = h.code[i];
+      const int32_t cnt = __popcll(code);
+      const bool kept = occ && cnt >= a.min_nnz;
+      const uint64_t m = __ballot(occ);
+      const int32_t j = D + __popcll(m & below_lane);
+      if (occ) {
+        h.jof[i] = j | (kept ? 0 : 256);
+        o.wkey[cE0 + j] = k;
+        o.wcode[cE0 + j] = code;
       }
       D += __popcll(m);
+      nt += __popcll(__ballot(kept));
+      nv += kept ? cnt : 0;
     }
-    // rank of each key = number of smaller keys (keys are unique); broadcast reads, no
-    // stage barriers
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) nv += __shfl_xor(nv, m, kWave);
+    if (lane == 0) {
+      o.nt[s] = nt;
+      o.nv[s] = nv;
+      o.nd[s] = D;
+    }
     wsync();
-    {
-      int32_t mykey[kHashCap / kWave], myslot[kHashCap / kWave], rank[kHashCap / kWave];
+
+    // ---- entries: block index and bit of each, rejected entries per row
+    int32_t nrow = 0;
+    auto sweep = [&](int32_t c, int32_t col, int32_t inf) {
+      const int32_t e = c + lane;
+      const bool valid = e < cE1;
+      int slot = 0, bit = 0;
+      if (valid) {
+        if (inf >= 0) {
+          slot = inf & 255;
+          bit = inf >> 8;
+        } else {
+          const int r = chunk_row(c);
+          const int32_t b = block_of(col);
+          bit = bit_of(r, col, b);
+          slot = hash_slot(b);
+          while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
+        }
+      }
+      const int32_t x = valid ? h.jof[slot] : 0;
+      if (valid) o.winfo[e] = (uint16_t)((x & 255) | (bit << 8));
+      const uint64_t rm = __ballot(valid && (x & 256) != 0);
+      if (lane < br) {
+        const int lo = min(max(rk - c, 0), kWave), hi = min(max(rk1 - c, 0), kWave);
+        nrow += __popcll(rm & low_bits(hi) & ~low_bits(lo));
+      }
+    };
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t c = cE0 + k * kWave;
+      if (c < cE1) sweep(c, 0, pinf[k]);
+    }
+    for (int32_t c = cE0 + kHashPre * kWave; c < cE1; c += kWave) {
+      const int32_t e = c + lane;
+      sweep(c, e < cE1 ? a.indices[e] : 0, -1);
+    }
+    if (lane < br && r0 + lane < a.rows) o.nr[r0 + lane] = nrow;
+    wsync();
+  }
+}
+
+// 7 waves per SIMD (72 VGPRs), which is also what its LDS allows (7 workgroups per CU).
+constexpr int kHashFillWavesPerSimd = 7;
+template <typename V, bool P2>
+__global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildArgs a, FillOut o) {
+  __shared__ FillLds sh_all[4];
+  const int lane = threadIdx.x & (kWave - 1);
+  FillLds& h = sh_all[threadIdx.x / kWave];
+  const int64_t wave = __builtin_amdgcn_readfirstlane(
+      (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  const V* __restrict__ vals = (const V*)o.values;
+  V* __restrict__ data = (V*)o.data;
+  V* __restrict__ rem_val = (V*)o.rem_val;
+  const int br = a.br, bc = a.bc;
+  const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
+  const uint64_t below_lane = low_bits(lane);
+  const bool all_blocks = o.blk_bc != nullptr;
+  // a stripe's scalars, one per lane of lanes 0..7, issued a stripe ahead: first and end
+  // entry, tile / value / distinct-block offsets, next distinct-block offset, remainder
+  // offset, overflow flag
+  auto load_head = [&](int64_t st) {
+    int32_t v = 0;
+    if (st < a.num_stripes) {
+      const int64_t rb = st * br;
+      if (lane == 0) v = a.indptr[rb];
+      if (lane == 1) v = a.indptr[min<int64_t>(rb + br, a.rows)];
+      if (lane == 2) v = o.nt[st];
+      if (lane == 3) v = o.nv[st];
+      if (lane == 4) v = o.nd[st];
+      if (lane == 5) v = o.nd[st + 1];
+      if (lane == 6) v = o.nr[rb];
+      if (lane == 7) v = o.big[st];
+    }
+    return v;
+  };
+  int32_t head = load_head(wave);
+  // the last wave closes the offset arrays: offsets[T] = V, stripe_ptr[ns] = T
+  if (wave == nwaves - 1 && lane == 0) {
+    const int32_t T = o.nt[a.num_stripes];
+    o.offsets[T] = o.nv[a.num_stripes];
+    if (o.stripe_ptr) o.stripe_ptr[a.num_stripes] = T;
+  }
+
+  for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
+    const int64_t r0 = s * br;
+    const int32_t E0 = __builtin_amdgcn_readlane(head, 0);
+    const int32_t E1 = __builtin_amdgcn_readlane(head, 1);
+    const int32_t tile_base = __builtin_amdgcn_readlane(head, 2);
+    const int32_t val_base = __builtin_amdgcn_readlane(head, 3);
+    const int32_t d_base = __builtin_amdgcn_readlane(head, 4);
+    const int32_t D = __builtin_amdgcn_readlane(head, 5) - d_base;
+    const int32_t rem_base = __builtin_amdgcn_readlane(head, 6);
+    const int32_t big = __builtin_amdgcn_readlane(head, 7);
+    if (lane == 0 && o.stripe_ptr) o.stripe_ptr[s] = tile_base;
+    if (big) {  // the merge kernel fills this stripe
+      head = load_head(s + nwaves);
+      continue;
+    }
+    // block list and the first chunks of entries, then the next stripe's scalars
+    int32_t bkey[kHashCap / kWave];
+    uint64_t bcode[kHashCap / kWave];
+#pragma unroll
+    for (int t = 0; t < kHashCap / kWave; ++t) {
+      const int j = t * kWave + lane;
+      bkey[t] = 0;
+      bcode[t] = 0ull;
+      if (t * kWave < D && j < D) {
+        bkey[t] = o.wkey[E0 + j];
+        bcode[t] = o.wcode[E0 + j];
+      }
+    }
+    int32_t pinf[kHashPre];
+    V pval[kHashPre];
+#pragma unroll
+    for (int k = 0; k < kHashPre; ++k) {
+      const int32_t e = E0 + k * kWave + lane;
+      pinf[k] = e < E1 ? (int32_t)o.winfo[e] : 0;
+      pval[k] = e < E1 ? vals[e] : V(0);
+    }
+    head = load_head(s + nwaves);
+
+    // ---- listed blocks: the kept ones, or every block when the reference-order block list
+    // is written
+    int32_t L = 0;
+#pragma unroll
+    for (int t = 0; t < kHashCap / kWave; ++t) {
+      if (t * kWave >= D) break;
+      const int j = t * kWave + lane;
+      const bool valid = j < D;
+      const bool listed = valid && (all_blocks || __popcll(bcode[t]) >= a.min_nnz);
+      if (valid) {
+        h.key[j] = bkey[t];
+        h.code[j] = bcode[t];
+        h.voff[j] = -1;
+      }
+      const uint64_t m = __ballot(listed);
+      if (listed) {
+        const int pos = L + __popcll(m & below_lane);
+        h.lkey[pos] = bkey[t];
+        h.lidx[pos] = (uint8_t)j;
+      }
+      L += __popcll(m);
+    }
+    wsync();
+    // rank of each listed block = number of smaller block columns (unique)
+    if (L <= kWave) {  // one listed block per lane; the keys come by lane reads
+      const int32_t mykey = lane < L ? h.lkey[lane] : INT32_MAX;
+      const int32_t myj = lane < L ? h.lidx[lane] : 0;
+      int32_t rank = 0;
+      for (int i = 0; i < L; ++i) rank += __builtin_amdgcn_readlane(mykey, i) < mykey ? 1 : 0;
+      wsync();
+      if (lane < L) h.lidx[rank] = (uint8_t)myj;
+    } else {  // broadcast LDS reads
+      int32_t mykey[kHashCap / kWave], myj[kHashCap / kWave], rank[kHashCap / kWave];
 #pragma unroll
       for (int t = 0; t < kHashCap / kWave; ++t) {
-        const int j = t * kWave + lane;
-        mykey[t] = j < D ? h.voff[j] : INT32_MAX;
-        myslot[t] = j < D ? h.sslot[j] : -1;
+        const int q = t * kWave + lane;
+        mykey[t] = q < L ? h.lkey[q] : INT32_MAX;
+        myj[t] = q < L ? h.lidx[q] : 0;
         rank[t] = 0;
       }
-      for (int i = 0; i < D; ++i) {
-        const int32_t ki = h.voff[i];
+      for (int i = 0; i < L; ++i) {
+        const int32_t ki = h.lkey[i];
 #pragma unroll
         for (int t = 0; t < kHashCap / kWave; ++t) rank[t] += ki < mykey[t] ? 1 : 0;
       }
       wsync();
 #pragma unroll
-      for (int t = 0; t < kHashCap / kWave; ++t) {
-        if (t * kWave + lane < D) h.sslot[rank[t]] = myslot[t];
-        // kept-only list: every unlisted block is rejected (the listed ones are set below)
-        if (!all_blocks) h.voff[t * kWave + lane] = -1;
-      }
+      for (int t = 0; t < kHashCap / kWave; ++t)
+        if (t * kWave + lane < L) h.lidx[rank[t]] = (uint8_t)myj[t];
     }
     wsync();
 
-    // ---- kept blocks: tile indices and value offsets by prefix sums over the sorted list
-    int32_t tile_base = 0, val_base = 0, d_base = 0;
-    if (FILL) {
-      tile_base = cur.nt;
-      val_base = cur.nv;
-      d_base = cur.nd;
-    }
+    // ---- tile indices and value offsets by prefix sums in block-column order
     int32_t kcar = 0, vcar = 0;
-    for (int j0 = 0; j0 < D; j0 += kWave) {
-      const int j = j0 + lane;
-      const bool valid = j < D;
-      const int slot = valid ? h.sslot[j] : 0;
-      const uint64_t code = valid ? (((uint64_t)h.hi[slot] << 32) | h.lo[slot]) : 0ull;
+    for (int q0 = 0; q0 < L; q0 += kWave) {
+      const int q = q0 + lane;
+      const bool valid = q < L;
+      const int j = valid ? h.lidx[q] : 0;
+      const uint64_t code = valid ? h.code[j] : 0ull;
+      const int32_t key = h.key[j];
       const int32_t cnt = __popcll(code);
       const bool kept = valid && cnt >= a.min_nnz;
       const uint64_t km = __ballot(kept);
@@ -473,86 +633,54 @@ __global__ void __launch_bounds__(256, FILL ? kHashWavesPerSimd : 1)
         if (lane >= d) vin += y;
       }
       const int32_t vex = vcar + vin - (kept ? cnt : 0);
-      if (valid) {
-        const int32_t t = kept ? (FILL ? tile_base + krank : 1) : -1;
-        h.voff[slot] = kept ? val_base + vex : -1;
-        if (FILL) {
-          const int32_t key = h.key[slot];
-          if (kept) {
-            o.codes[t] = code;
-            o.coords[2 * (int64_t)t] = (int32_t)r0;
-            o.coords[2 * (int64_t)t + 1] = key * bc;
-            o.offsets[t] = val_base + vex;
-          }
-          if (o.blk_bc != nullptr) {
-            o.blk_bc[d_base + j] = key;
-            o.blk_rmin[d_base + j] = (int8_t)(__builtin_ctzll(code) / bc);
-            o.blk_tile[d_base + j] = kept ? t : -1;
-          }
-        }
+      if (kept) {
+        const int32_t t = tile_base + krank;
+        h.voff[j] = val_base + vex;
+        o.codes[t] = code;
+        o.coords[2 * (int64_t)t] = (int32_t)r0;
+        o.coords[2 * (int64_t)t + 1] = key * bc;
+        o.offsets[t] = val_base + vex;
+      }
+      if (all_blocks && valid) {
+        o.blk_bc[d_base + q] = key;
+        o.blk_rmin[d_base + q] = (int8_t)(__builtin_ctzll(code) / bc);
+        o.blk_tile[d_base + q] = kept ? tile_base + krank : -1;
       }
       kcar += __popcll(km);
       vcar += __shfl(vin, kWave - 1, kWave);
     }
-    }  // FILL
     wsync();
 
-    // ---- entries: kept values into their tile, rejected ones to the remainder row
-    // inf: what insert returned for the entry, or -1 to find its slot again
-    auto sweep = [&](int32_t c, int32_t col, V val, int32_t inf) {
+    // ---- entries: kept values into their tile, rejected ones to the remainder
+    int32_t carry = 0;  // rejected entries of the stripe so far
+    auto sweep = [&](int32_t c, int32_t inf, V val) {
       const int32_t e = c + lane;
       const bool valid = e < E1;
-      int r = 0, slot = 0, bit = 0;
-      if (valid) {
-        if (inf >= 0) {
-          slot = inf & 255;
-          bit = (inf >> 8) & 63;
-          r = inf >> 14;
-        } else {
-          r = row_at(e);
-          const int32_t b = block_of(col);
-          bit = bit_of(r, col, b);
-          slot = hash_slot(b);
-          while (h.key[slot] != b) slot = (slot + 1) & (kHashCap - 1);
-        }
-      }
-      const bool rej = valid && h.voff[slot] < 0;
+      const int j = inf & 255;
+      const int bit = inf >> 8;
+      const int32_t vo = valid ? h.voff[j] : 0;
+      const bool rej = valid && vo < 0;
       const uint64_t rm = __ballot(rej);
-      // lanes of this row in this chunk: [ls, ...]
-      const int ls = valid ? max(0, h.rows[r] - c) : 0;
-      const uint64_t from_ls = ls == 0 ? ~0ull : ~(~0ull >> (64 - ls));
-      const int32_t before = __popcll(rm & below_lane & from_ls);
-      if constexpr (FILL) {
-        if (valid) {
-          if (rej) {
-            const int32_t pos = h.rcnt[r] + before;
-            o.rem_col[pos] = col;
-            ((V*)o.rem_val)[pos] = val;
-          } else {
-            const uint64_t code = ((uint64_t)h.hi[slot] << 32) | h.lo[slot];
-            const uint64_t below = bit == 0 ? 0ull : (~0ull >> (64 - bit));
-            ((V*)o.data)[h.voff[slot] + __popcll(code & below)] = val;
-          }
-        }
+      if (rej) {
+        const int32_t pos = rem_base + carry + __popcll(rm & below_lane);
+        const int32_t in_blk = P2 ? (bit & (bc - 1)) : bit % bc;
+        o.rem_col[pos] = h.key[j] * bc + in_blk;
+        rem_val[pos] = val;
+      } else if (valid) {
+        data[vo + __popcll(h.code[j] & low_bits(bit))] = val;
       }
-      // the last lane of each row in this chunk advances that row's count
-      const bool last = valid && (lane == kWave - 1 || e + 1 == h.rows[r + 1]);
-      const int32_t in_row = __popcll(rm & (below_lane | (1ull << lane)) & from_ls);
-      wsync();
-      if (last) h.rcnt[r] += in_row;
-      wsync();
+      carry += __popcll(rm);
     };
 #pragma unroll
     for (int k = 0; k < kHashPre; ++k) {
       const int32_t c = E0 + k * kWave;
-      if (c < E1) sweep(c, pcol[k], FILL ? pval[k] : V(0), pinf[k]);
+      if (c < E1) sweep(c, pinf[k], pval[k]);
     }
     for (int32_t c = E0 + kHashPre * kWave; c < E1; c += kWave) {
       const int32_t e = c + lane;
       const bool ok = e < E1;
-      sweep(c, ok ? a.indices[e] : 0, (FILL && ok) ? vals[e] : V(0), -1);
+      sweep(c, ok ? (int32_t)o.winfo[e] : 0, ok ? vals[e] : V(0));
     }
-    if (!FILL && lane < br && r0 + lane < a.rows) o.nr[r0 + lane] = h.rcnt[lane];
     wsync();
   }
 }
@@ -653,28 +781,157 @@ int excl_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, hipStrea
   return GL_OK;
 }
 
+// ---- the builder's four scans (tiles, tile values and distinct blocks per stripe,
+// remainder entries per row) in two launches: tile-local scans of all four arrays, then one
+// that adds each tile's carry (the sum of the tile sums before it, read by the tile itself)
+// and writes out[n] and the four totals next to each other for a single copy to the host.
+// Arrays of more than kScan4MaxTiles tiles take the three-launch scan above instead.
+constexpr int kScan4MaxTiles = 2048;
+struct Scan4 {
+  int32_t* a[4];
+  int64_t n[4];
+  int32_t* bsum[4];
+  int32_t tiles[4];
+  int32_t first[5];   // first workgroup of array i in the tile launch
+  int32_t afirst[5];  // ... in the add launch (at least one workgroup per array)
+  int32_t* totals;
+};
+
+__device__ __forceinline__ int scan4_array(const int32_t* first, int32_t blk) {
+  int i = 0;
+  while (i < 3 && blk >= first[i + 1]) ++i;
+  return i;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan4_tiles(Scan4 p) {
+  __shared__ int32_t sh[65];
+  const int i = scan4_array(p.first, (int32_t)blockIdx.x);
+  const int64_t j = blockIdx.x - p.first[i];
+  int32_t* __restrict__ a = p.a[i];
+  const int64_t n = p.n[i];
+  const int64_t base = j * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? a[base + k] : 0;
+    s += v[k];
+  }
+  int32_t total;
+  int32_t run = block_excl_scan(s, sh, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) a[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) p.bsum[i][j] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan4_add(Scan4 p) {
+  __shared__ int32_t sh[65];
+  const int i = scan4_array(p.afirst, (int32_t)blockIdx.x);
+  const int64_t j = blockIdx.x - p.afirst[i];
+  int32_t* __restrict__ a = p.a[i];
+  const int64_t n = p.n[i];
+  const int32_t* __restrict__ bs = p.bsum[i];
+  int32_t s = 0;
+  for (int64_t k = threadIdx.x; k < j; k += kScanThreads) s += bs[k];
+  int32_t carry;
+  (void)block_excl_scan(s, sh, &carry);
+  if (carry != 0) {
+    const int64_t base = j * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t idx = base + (int64_t)k * kScanThreads + threadIdx.x;
+      if (idx < n) a[idx] += carry;
+    }
+  }
+  if (threadIdx.x == 0 && j + 1 >= p.tiles[i]) {
+    const int32_t total = carry + (p.tiles[i] > 0 ? bs[j] : 0);
+    a[n] = total;
+    p.totals[i] = total;
+  }
+}
+
+__global__ void k_scan4_totals(int32_t* a0, int64_t n0, int32_t* a1, int64_t n1, int32_t* a2,
+                               int64_t n2, int32_t* a3, int64_t n3, int32_t* totals) {
+  if (threadIdx.x == 0) {
+    totals[0] = a0[n0];
+    totals[1] = a1[n1];
+    totals[2] = a2[n2];
+    totals[3] = a3[n3];
+  }
+}
+
+// In-place exclusive scans of a[i][0..n[i]) with a[i][n[i]] = total, totals[i] = total.
+// bsum: kScan4MaxTiles ints per array, or scan_tmp_elems() for the three-launch scan.
+int scan4(int32_t* const a[4], const int64_t n[4], int32_t* bsum, int32_t* totals,
+          hipStream_t st) {
+  Scan4 p{};
+  int64_t max_tiles = 0;
+  for (int i = 0; i < 4; ++i) max_tiles = std::max(max_tiles, (n[i] + kScanTile - 1) / kScanTile);
+  if (max_tiles > kScan4MaxTiles) {
+    for (int i = 0; i < 4; ++i) {
+      int rc = excl_scan(a[i], a[i], n[i], bsum, st);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_scan4_totals, dim3(1), dim3(64), 0, st, a[0], n[0], a[1], n[1], a[2],
+                       n[2], a[3], n[3], totals);
+    GL_LAUNCH_CHECK("k_scan4_totals");
+    return GL_OK;
+  }
+  p.first[0] = p.afirst[0] = 0;
+  for (int i = 0; i < 4; ++i) {
+    p.a[i] = a[i];
+    p.n[i] = n[i];
+    p.bsum[i] = bsum + (int64_t)i * kScan4MaxTiles;
+    p.tiles[i] = (int32_t)((n[i] + kScanTile - 1) / kScanTile);
+    p.first[i + 1] = p.first[i] + p.tiles[i];
+    p.afirst[i + 1] = p.afirst[i] + std::max(p.tiles[i], 1);
+  }
+  p.totals = totals;
+  if (p.first[4] > 0) {
+    hipLaunchKernelGGL(k_scan4_tiles, dim3((unsigned)p.first[4]), dim3(kScanThreads), 0, st, p);
+    GL_LAUNCH_CHECK("k_scan4_tiles");
+  }
+  hipLaunchKernelGGL(k_scan4_add, dim3((unsigned)p.afirst[4]), dim3(kScanThreads), 0, st, p);
+  GL_LAUNCH_CHECK("k_scan4_add");
+  return GL_OK;
+}
+
 // ---- workspace layout ----
+// Per stripe: counts / offsets (nt, nv, nd), overflow flags; scan scratch; the four totals;
+// per CSR entry (count pass -> fill pass): block lists (code, block column) at the stripe's
+// first entry and each entry's block index and bit.
 struct BuildWs {
-  int32_t *nt, *nv, *nd, *tmp;
+  int32_t *nt, *nv, *nd, *tmp, *tot;
   uint8_t* big;
+  uint64_t* wcode;
+  int32_t* wkey;
+  uint16_t* winfo;
   int64_t bytes;
 };
 static int64_t al(int64_t x) { return (x + 255) & ~(int64_t)255; }
-static BuildWs carve_build(char* base, int64_t rows, int32_t br) {
+static BuildWs carve_build(char* base, int64_t rows, int32_t br, int64_t nnz) {
   const int64_t ns = (rows + br - 1) / br;
-  const int64_t tmpn = scan_tmp_elems(std::max(rows, ns) + 1);
+  const int64_t tmpn = std::max<int64_t>(scan_tmp_elems(std::max(rows, ns) + 1),
+                                         4 * (int64_t)kScan4MaxTiles);
   BuildWs w;
   int64_t off = 0;
   auto take = [&](int64_t n) {
     char* p = base ? base + off : nullptr;
     off += al(n);
-    return (int32_t*)p;
+    return (void*)p;
   };
-  w.nt = take((ns + 1) * 4);
-  w.nv = take((ns + 1) * 4);
-  w.nd = take((ns + 1) * 4);
-  w.tmp = take(tmpn * 4);
+  w.nt = (int32_t*)take((ns + 1) * 4);
+  w.nv = (int32_t*)take((ns + 1) * 4);
+  w.nd = (int32_t*)take((ns + 1) * 4);
+  w.tmp = (int32_t*)take(tmpn * 4);
+  w.tot = (int32_t*)take(4 * 4);
   w.big = (uint8_t*)take(ns);
+  w.wcode = (uint64_t*)take(nnz * 8);
+  w.wkey = (int32_t*)take(nnz * 4);
+  w.winfo = (uint16_t*)take(nnz * 2);
   w.bytes = off;
   return w;
 }
@@ -685,15 +942,15 @@ static int group_lanes(int br) {
   return g;
 }
 
+// Waves: the count pass uses kHashCountWgPerCu workgroups per CU, the fill pass fills the
+// CUs once (its occupancy, from its LDS).
 template <bool FILL, typename V, bool P2>
 static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t st) {
-  // waves: the fill pass fills the CUs once (fewer or more workgroups were slower), the
-  // count pass uses kHashCountWgPerCu
   static std::atomic<int> per_cu{-1};
   if (per_cu.load() < 0) {
     int n = kHashCountWgPerCu;
     if (FILL &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_build_hash<FILL, V, P2>, 256, 0) !=
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hash_fill<V, P2>, 256, 0) !=
              hipSuccess ||
          n < 1))
       n = 4;
@@ -701,8 +958,12 @@ static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t 
   }
   const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
   const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
-  hipLaunchKernelGGL((k_build_hash<FILL, V, P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
-                     st, a, o);
+  if constexpr (FILL)
+    hipLaunchKernelGGL((k_hash_fill<V, P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st,
+                       a, o);
+  else
+    hipLaunchKernelGGL((k_hash_count<P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st, a,
+                       o);
 }
 
 template <bool FILL, typename V>
@@ -710,7 +971,7 @@ static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
   if (o.big != nullptr) {
     if ((a.bc & (a.bc - 1)) == 0) launch_build_hash<FILL, V, true>(a, o, st);
     else launch_build_hash<FILL, V, false>(a, o, st);
-    GL_LAUNCH_CHECK("k_build_hash");
+    GL_LAUNCH_CHECK("k_hash_count / k_hash_fill");
   }
   const int GLn = group_lanes(a.br);
   const int spw = kWave / GLn;
@@ -785,15 +1046,16 @@ using namespace gl;
 
 extern "C" {
 
-int64_t gl_pbr_build_workspace_bytes(int64_t rows, int32_t block_rows) {
-  if (rows < 0 || block_rows < 1) return -1;
-  return carve_build(nullptr, rows, block_rows).bytes;
+int64_t gl_pbr_build_workspace_bytes(int64_t rows, int32_t block_rows, int64_t nnz) {
+  if (rows < 0 || block_rows < 1 || nnz < 0) return -1;
+  return carve_build(nullptr, rows, block_rows, nnz).bytes;
 }
 
-static int check_build(int64_t rows, int64_t cols, int32_t br, int32_t bc, int32_t min_nnz,
-                       const int32_t* indptr, const int32_t* indices) {
-  GL_REQUIRE(rows >= 0 && cols >= 0, "negative size");
-  GL_REQUIRE(rows < INT32_MAX && cols < INT32_MAX, "rows/cols exceed int32 indices");
+static int check_build(int64_t rows, int64_t cols, int64_t nnz, int32_t br, int32_t bc,
+                       int32_t min_nnz, const int32_t* indptr, const int32_t* indices) {
+  GL_REQUIRE(rows >= 0 && cols >= 0 && nnz >= 0, "negative size");
+  GL_REQUIRE(rows < INT32_MAX && cols < INT32_MAX && nnz < INT32_MAX,
+             "rows/cols/nnz exceed int32 indices");
   GL_REQUIRE(br >= 1 && bc >= 1 && (int64_t)br * bc <= 64,
              "block %dx%d does not fit a 64-bit code", br, bc);
   GL_REQUIRE(min_nnz >= 0 && min_nnz <= 255, "min_nnz_per_block must be in [0, 255]");
@@ -802,20 +1064,21 @@ static int check_build(int64_t rows, int64_t cols, int32_t br, int32_t bc, int32
   return GL_OK;
 }
 
-int gl_pbr_build_count(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                       int32_t min_nnz_per_block, const int32_t* indptr,
+int gl_pbr_build_count(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
+                       int32_t block_cols, int32_t min_nnz_per_block, const int32_t* indptr,
                        const int32_t* indices, int32_t* rem_indptr, void* workspace,
                        int64_t workspace_bytes, int64_t* counts, void* stream) {
-  int rc = check_build(rows, cols, block_rows, block_cols, min_nnz_per_block, indptr, indices);
+  int rc = check_build(rows, cols, nnz, block_rows, block_cols, min_nnz_per_block, indptr,
+                       indices);
   if (rc) return rc;
   GL_REQUIRE(rem_indptr && counts, "null output");
-  const int64_t need = carve_build(nullptr, rows, block_rows).bytes;
+  const int64_t need = carve_build(nullptr, rows, block_rows, nnz).bytes;
   if (!workspace || workspace_bytes < need) {
     set_error("build workspace too small (%lld < %lld)", (long long)workspace_bytes,
               (long long)need);
     return GL_ERR_WORKSPACE;
   }
-  BuildWs w = carve_build((char*)workspace, rows, block_rows);
+  BuildWs w = carve_build((char*)workspace, rows, block_rows, nnz);
   hipStream_t st = (hipStream_t)stream;
   const int64_t ns = (rows + block_rows - 1) / block_rows;
   BuildArgs a{rows, cols, block_rows, block_cols, min_nnz_per_block, ns, indptr, indices};
@@ -824,41 +1087,42 @@ int gl_pbr_build_count(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
   o.nv = w.nv;
   o.nr = rem_indptr;
   o.nd = w.nd;
-  o.big = w.big;
-  if (ns > 0) GL_HIP(hipMemsetAsync(w.big, 0, ns, st));
+  o.big = w.big;  // written for every stripe by the count pass
+  o.wcode = w.wcode;
+  o.wkey = w.wkey;
+  o.winfo = w.winfo;
   if (ns > 0 && (rc = launch_build<false, uint32_t>(a, o, st))) return rc;
-  if ((rc = excl_scan(w.nt, w.nt, ns, w.tmp, st))) return rc;
-  if ((rc = excl_scan(w.nv, w.nv, ns, w.tmp, st))) return rc;
-  if ((rc = excl_scan(w.nd, w.nd, ns, w.tmp, st))) return rc;
-  if ((rc = excl_scan(rem_indptr, rem_indptr, rows, w.tmp, st))) return rc;
-  int32_t h[4];
-  GL_HIP(hipMemcpyAsync(&h[0], w.nt + ns, 4, hipMemcpyDeviceToHost, st));
-  GL_HIP(hipMemcpyAsync(&h[1], w.nv + ns, 4, hipMemcpyDeviceToHost, st));
-  GL_HIP(hipMemcpyAsync(&h[2], rem_indptr + rows, 4, hipMemcpyDeviceToHost, st));
-  GL_HIP(hipMemcpyAsync(&h[3], w.nd + ns, 4, hipMemcpyDeviceToHost, st));
+  int32_t* arrs[4] = {w.nt, w.nv, rem_indptr, w.nd};
+  const int64_t lens[4] = {ns, ns, rows, ns};
+  if ((rc = scan4(arrs, lens, w.tmp, w.tot, st))) return rc;
+  // one copy of the four totals into pinned memory of this thread
+  static thread_local int32_t* h_tot = nullptr;
+  if (!h_tot) GL_HIP(hipHostMalloc((void**)&h_tot, 4 * sizeof(int32_t), hipHostMallocDefault));
+  GL_HIP(hipMemcpyAsync(h_tot, w.tot, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   GL_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < 4; ++i) counts[i] = h[i];
+  for (int i = 0; i < 4; ++i) counts[i] = h_tot[i];
   return GL_OK;
 }
 
-int gl_pbr_build_fill(int64_t rows, int64_t cols, int32_t block_rows, int32_t block_cols,
-                      int32_t min_nnz_per_block, int64_t num_tiles, const int32_t* indptr,
-                      const int32_t* indices,
-                      const void* values, int32_t value_bytes, void* workspace,
-                      int64_t workspace_bytes, uint64_t* codes, int32_t* coords,
-                      int32_t* offsets, void* data, const int32_t* rem_indptr,
-                      int32_t* rem_col_ind, void* rem_vals, int32_t* blk_ptr, int32_t* blk_bc,
-                      int8_t* blk_rmin, int32_t* blk_tile, void* stream) {
-  int rc = check_build(rows, cols, block_rows, block_cols, min_nnz_per_block, indptr, indices);
+int gl_pbr_build_fill(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
+                      int32_t block_cols, int32_t min_nnz_per_block, int64_t num_tiles,
+                      const int32_t* indptr, const int32_t* indices, const void* values,
+                      int32_t value_bytes, void* workspace, int64_t workspace_bytes,
+                      uint64_t* codes, int32_t* coords, int32_t* offsets, void* data,
+                      const int32_t* rem_indptr, int32_t* rem_col_ind, void* rem_vals,
+                      int32_t* stripe_ptr, int32_t* blk_ptr, int32_t* blk_bc, int8_t* blk_rmin,
+                      int32_t* blk_tile, void* stream) {
+  int rc = check_build(rows, cols, nnz, block_rows, block_cols, min_nnz_per_block, indptr,
+                       indices);
   if (rc) return rc;
   GL_REQUIRE(value_bytes == 4 || value_bytes == 8, "value_bytes must be 4 or 8");
   GL_REQUIRE(offsets && rem_indptr, "null output");
-  const int64_t need = carve_build(nullptr, rows, block_rows).bytes;
+  const int64_t need = carve_build(nullptr, rows, block_rows, nnz).bytes;
   if (!workspace || workspace_bytes < need) {
     set_error("build workspace too small");
     return GL_ERR_WORKSPACE;
   }
-  BuildWs w = carve_build((char*)workspace, rows, block_rows);
+  BuildWs w = carve_build((char*)workspace, rows, block_rows, nnz);
   hipStream_t st = (hipStream_t)stream;
   const int64_t ns = (rows + block_rows - 1) / block_rows;
   BuildArgs a{rows, cols, block_rows, block_cols, min_nnz_per_block, ns, indptr, indices};
@@ -878,13 +1142,20 @@ int gl_pbr_build_fill(int64_t rows, int64_t cols, int32_t block_rows, int32_t bl
   o.blk_rmin = blk_rmin;
   o.blk_tile = blk_tile;
   o.big = w.big;  // as the count pass left it on this workspace
+  o.stripe_ptr = stripe_ptr;
+  o.wcode = w.wcode;
+  o.wkey = w.wkey;
+  o.winfo = w.winfo;
   if (ns > 0) {
+    // the fill kernel also writes offsets[T] = V and stripe_ptr[ns] = T
     rc = value_bytes == 4 ? launch_build<true, uint32_t>(a, o, st)
                           : launch_build<true, uint64_t>(a, o, st);
     if (rc) return rc;
+  } else {
+    GL_HIP(hipMemcpyAsync(offsets + num_tiles, w.nv, 4, hipMemcpyDeviceToDevice, st));
+    if (stripe_ptr) GL_HIP(hipMemcpyAsync(stripe_ptr, w.nt, 4, hipMemcpyDeviceToDevice, st));
   }
-  // offsets[T] = V, and the distinct-block pointer per stripe when asked for
-  GL_HIP(hipMemcpyAsync(offsets + num_tiles, w.nv + ns, 4, hipMemcpyDeviceToDevice, st));
+  // the distinct-block pointer per stripe when asked for
   if (blk_ptr) GL_HIP(hipMemcpyAsync(blk_ptr, w.nd, (ns + 1) * 4, hipMemcpyDeviceToDevice, st));
   return GL_OK;
 }

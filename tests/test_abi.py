@@ -40,7 +40,7 @@ def test_ctypes_table_matches_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 8
+    assert L.gl_abi_version() == 9
     assert L.gl_last_error() is not None
 
 
@@ -58,8 +58,11 @@ def test_argument_validation_without_device():
     assert rc == _native.GL_ERR_INVALID
     assert L.gl_ppr_workspace_bytes(100, 8, 4, 3) > 100 * 4 * 4
     assert L.gl_ppr_workspace_bytes(-1, 8, 4, 3) == -1
-    assert L.gl_pbr_build_workspace_bytes(100, 8) > 0
-    rc = L.gl_pbr_build_count(10, 10, 9, 9, 1, None, None, None, None, 0, None, None)
+    assert L.gl_pbr_build_workspace_bytes(100, 8, 0) > 0
+    assert L.gl_pbr_build_workspace_bytes(100, 8, 1000) >= \
+        L.gl_pbr_build_workspace_bytes(100, 8, 0) + 14 * 1000
+    assert L.gl_pbr_build_workspace_bytes(100, 8, -1) == -1
+    rc = L.gl_pbr_build_count(10, 10, 0, 9, 9, 1, None, None, None, None, 0, None, None)
     assert rc == _native.GL_ERR_INVALID and "64-bit code" in _native.last_error()
 
 
