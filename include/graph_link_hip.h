@@ -193,9 +193,11 @@ int gl_ppr_run_i32_f32(int64_t rows, int64_t cols, int32_t block_rows, int32_t b
  * per-stripe counts plus 14 bytes per CSR entry, which carry each entry's block and bit from
  * the count pass to the fill pass so that the fill pass neither reads column indices nor
  * hashes):
- *   gl_pbr_build_count: fills rem_indptr[rows+1] and returns counts[4] = {T, V, R, D}
- *                       (tiles, tile values, remainder entries, distinct blocks) on the host;
- *   gl_pbr_build_fill : writes codes[T], coords[2T], offsets[T+1], data[V], rem_col_ind[R],
+ *   gl_pbr_build_count: fills rem_indptr[rows+1] and returns counts[5] = {T, V, R, D, B}
+ *                       (tiles, tile values, remainder entries, distinct blocks, stripes
+ *                       with more distinct blocks than the hash table holds) on the host;
+ *   gl_pbr_build_fill : given T and B (B < 0: unknown), writes codes[T], coords[2T],
+ *                       offsets[T+1], data[V], rem_col_ind[R],
  *                       rem_vals[R] in canonical order (block columns ascending inside a
  *                       stripe), stripe_ptr[ns+1] (first tile of each stripe) when not NULL,
  *                       and, when blk_bc != NULL, the distinct-block list
@@ -210,7 +212,8 @@ int gl_pbr_build_count(int64_t rows, int64_t cols, int64_t nnz, int32_t block_ro
                        int64_t workspace_bytes, int64_t* counts, void* stream);
 int gl_pbr_build_fill(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
                       int32_t block_cols, int32_t min_nnz_per_block, int64_t num_tiles,
-                      const int32_t* indptr, const int32_t* indices, const void* values,
+                      int64_t num_big_stripes, const int32_t* indptr,
+                      const int32_t* indices, const void* values,
                       int32_t value_bytes, void* workspace, int64_t workspace_bytes,
                       uint64_t* codes, int32_t* coords, int32_t* offsets, void* data,
                       const int32_t* rem_indptr, int32_t* rem_col_ind, void* rem_vals,

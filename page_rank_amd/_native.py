@@ -77,9 +77,9 @@ _PROTOS = {
     "gl_pbr_build_workspace_bytes": (c_i64, [c_i64, c_i32, c_i64]),
     "gl_pbr_build_count": (ctypes.c_int, [c_i64, c_i64, c_i64, c_i32, c_i32, c_i32, c_p, c_p,
                                           c_p, c_p, c_i64, c_p, c_p]),
-    "gl_pbr_build_fill": (ctypes.c_int, [c_i64, c_i64, c_i64, c_i32, c_i32, c_i32, c_i64, c_p,
-                                         c_p, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p, c_p,
-                                         c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
+    "gl_pbr_build_fill": (ctypes.c_int, [c_i64, c_i64, c_i64, c_i32, c_i32, c_i32, c_i64, c_i64,
+                                         c_p, c_p, c_p, c_i32, c_p, c_i64, c_p, c_p, c_p, c_p,
+                                         c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "gl_pbr_reference_perm": (ctypes.c_int, [c_i64, c_i64, c_i32, c_i32, c_i32, c_p, c_p, c_p,
                                              c_p, c_p]),
     "gl_pbr_permute_workspace_bytes": (c_i64, [c_i64]),

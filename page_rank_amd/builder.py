@@ -87,37 +87,47 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
         wsb = L.gl_pbr_build_workspace_bytes(rows, block_rows, nnz)
         ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
         rem_indptr = torch.empty(rows + 1, dtype=torch.int32, device=dev)
-        counts = np.zeros(4, dtype=np.int64)
+        counts = np.zeros(5, dtype=np.int64)
         _native.check(L.gl_pbr_build_count(rows, cols, nnz, block_rows, block_cols,
                                            int(min_nnz_per_block), indptr.data_ptr(),
                                            indices.data_ptr(), rem_indptr.data_ptr(),
                                            ws.data_ptr(), wsb, counts.ctypes.data, stream),
                       "gl_pbr_build_count")
-        T, V, R, D = (int(c) for c in counts)
+        T, V, R, D, B = (int(c) for c in counts)
         ns = (rows + block_rows - 1) // block_rows
         vdt = torch.float64 if vb == 8 else torch.float32
-        codes = torch.empty(T, dtype=torch.int64, device=dev)
-        coords = torch.empty(2 * T, dtype=torch.int32, device=dev)
-        offsets = torch.empty(T + 1, dtype=torch.int32, device=dev)
-        data = torch.empty(V, dtype=vdt, device=dev)
-        rem_col = torch.empty(R, dtype=torch.int32, device=dev)
-        rem_vals = torch.empty(R, dtype=vdt, device=dev)
-        stripe_ptr = torch.empty(ns + 1, dtype=torch.int32, device=dev)
         want_blocks = order == "reference" and T > 1
+        # canonical order: every output in one device buffer, whose views are cut after the
+        # fill pass is queued (while it runs); the reference order replaces the tile arrays
+        # afterwards, so it allocates them one by one
+        shapes = [(T, torch.int64), (2 * T, torch.int32), (T + 1, torch.int32), (V, vdt),
+                  (R, torch.int32), (R, vdt), (ns + 1, torch.int32)]
         if want_blocks:
+            outs = [torch.empty(n, dtype=dt, device=dev) for n, dt in shapes]
+            ptrs = [o.data_ptr() for o in outs]
             blk_ptr = torch.empty(ns + 1, dtype=torch.int32, device=dev)
             blk_bc = torch.empty(max(D, 1), dtype=torch.int32, device=dev)
             blk_rmin = torch.empty(max(D, 1), dtype=torch.int8, device=dev)
             blk_tile = torch.empty(max(D, 1), dtype=torch.int32, device=dev)
         else:
+            starts, total = [], 0
+            for n, dt in shapes:
+                starts.append(total)
+                total += (n * dt.itemsize + 255) & ~255
+            buf = torch.empty(max(total, 1), dtype=torch.uint8, device=dev)
+            base = buf.data_ptr()
+            ptrs = [base + o for o in starts]
             blk_ptr = blk_bc = blk_rmin = blk_tile = None
         p = _native.ptr
         _native.check(L.gl_pbr_build_fill(
-            rows, cols, nnz, block_rows, block_cols, int(min_nnz_per_block), T,
+            rows, cols, nnz, block_rows, block_cols, int(min_nnz_per_block), T, B,
             indptr.data_ptr(), indices.data_ptr(), vals.data_ptr(), vb, ws.data_ptr(), wsb,
-            p(codes), p(coords), offsets.data_ptr(), p(data), rem_indptr.data_ptr(),
-            p(rem_col), p(rem_vals), stripe_ptr.data_ptr(), p(blk_ptr), p(blk_bc), p(blk_rmin),
-            p(blk_tile), stream), "gl_pbr_build_fill")
+            ptrs[0], ptrs[1], ptrs[2], ptrs[3], rem_indptr.data_ptr(), ptrs[4], ptrs[5],
+            ptrs[6], p(blk_ptr), p(blk_bc), p(blk_rmin), p(blk_tile), stream),
+            "gl_pbr_build_fill")
+        if not want_blocks:
+            outs = [buf[o:o + n * dt.itemsize].view(dt) for o, (n, dt) in zip(starts, shapes)]
+        codes, coords, offsets, data, rem_col, rem_vals, stripe_ptr = outs
         if want_blocks:
             h_ptr = blk_ptr.cpu().numpy()
             h_bc = blk_bc.cpu().numpy()

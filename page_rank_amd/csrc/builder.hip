@@ -74,13 +74,15 @@ struct FillOut {
   int32_t* blk_bc;    // optional block list for the reference-order replay
   int8_t* blk_rmin;
   int32_t* blk_tile;
-  uint8_t* big;       // stripes left to the merge kernel (more distinct blocks than the
-                      // LDS table holds); NULL = the merge kernel takes every stripe
+  int32_t* big;       // stripes left to the merge kernel (more distinct blocks than the
+                      // LDS table holds): flags [ns] in the count pass, their exclusive
+                      // prefix sums [ns+1] in the fill pass; NULL = the merge kernel takes
+                      // every stripe
   int32_t* stripe_ptr;  // optional: first tile of each stripe [ns+1]
   // count pass -> fill pass, indexed by CSR entry
-  uint64_t* wcode;    // block codes of a stripe's distinct blocks, from its first entry on
+  uint64_t* wcode;    // codes of a stripe's distinct blocks, from its first entry on
   int32_t* wkey;      // their block columns
-  uint16_t* winfo;    // per entry: block index | bit << 8
+  uint16_t* winfo;    // per entry: position of its block in the list | bit << 8
 };
 
 template <int GL, bool FILL, typename V>
@@ -99,7 +101,11 @@ __global__ void __launch_bounds__(256) k_build(BuildArgs a, FillOut o) {
   for (int64_t sb = wave * SPW; sb < a.num_stripes; sb += nwaves * SPW) {
     const int64_t s = sb + gidx;
     const int64_t row = s * a.br + r;
-    const bool mine = s < a.num_stripes && (o.big == nullptr || o.big[s] != 0);
+    // stripes the hash kernels left: flags in the count pass, their exclusive prefix sums
+    // (scanned with the counts) in the fill pass
+    const bool mine =
+        s < a.num_stripes &&
+        (o.big == nullptr || (FILL ? o.big[s + 1] != o.big[s] : o.big[s] != 0));
     const bool lane_ok = mine && r < a.br && row < a.rows;
     int32_t p = 0, pe = 0;
     if (lane_ok) {
@@ -231,12 +237,15 @@ struct CountLds {
   int32_t flag;           // table overflow
 };
 
+// Indexed by the block's position j in the stripe's block list.
 struct FillLds {
-  unsigned long long code[kHashCap];  // block j's code
-  int32_t key[kHashCap];              // block j's block column
-  int32_t voff[kHashCap];             // block j's first value offset when kept, else -1
-  int32_t lkey[kHashCap];             // listed blocks: block columns
-  uint8_t lidx[kHashCap];             // listed blocks: j, then j in ascending block column
+  unsigned long long code[kHashCap];  // block code
+  int32_t key[kHashCap];              // block column
+  union {
+    int32_t lkey[kHashCap];           // listed blocks: block columns (until ranked)
+    int32_t voff[kHashCap];           // then a block's first value offset when kept, else -1
+  };
+  uint8_t lidx[kHashCap];             // listed blocks: j, then j by block column
 };
 
 __device__ __forceinline__ void wsync() {
@@ -267,7 +276,7 @@ __device__ __forceinline__ int row_of(const int32_t* rows, int br, int32_t e) {
 // P2: block_cols is a power of two, so an entry's block column and bit come from a shift and
 // a mask instead of an integer division (both passes are bound by VALU issue).
 template <bool P2>
-__global__ void __launch_bounds__(256) k_hash_count(BuildArgs a, FillOut o) {
+__global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs a, FillOut o) {
   __shared__ CountLds sh_all[4];
   const int lane = threadIdx.x & (kWave - 1);
   CountLds& h = sh_all[threadIdx.x / kWave];
@@ -481,8 +490,8 @@ __global__ void __launch_bounds__(256) k_hash_count(BuildArgs a, FillOut o) {
   }
 }
 
-// 7 waves per SIMD (72 VGPRs), which is also what its LDS allows (7 workgroups per CU).
-constexpr int kHashFillWavesPerSimd = 7;
+// 8 waves per SIMD (64 VGPRs), which is also what its LDS allows (8 workgroups per CU).
+constexpr int kHashFillWavesPerSimd = 8;
 template <typename V, bool P2>
 __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildArgs a, FillOut o) {
   __shared__ FillLds sh_all[4];
@@ -498,25 +507,27 @@ __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildA
   const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
   const uint64_t below_lane = low_bits(lane);
   const bool all_blocks = o.blk_bc != nullptr;
-  // a stripe's scalars, one per lane of lanes 0..7, issued a stripe ahead: first and end
-  // entry, tile / value / distinct-block offsets, next distinct-block offset, remainder
-  // offset, overflow flag
+  // a stripe's scalars, scalar loads issued a stripe ahead: first and end entry, tile /
+  // value / distinct-block offsets, next distinct-block offset, remainder offset, overflow
+  struct Head {
+    int32_t E0, E1, tile, val, d0, d1, rem, big;
+  };
   auto load_head = [&](int64_t st) {
-    int32_t v = 0;
+    Head hd{0, 0, 0, 0, 0, 0, 0, 0};
     if (st < a.num_stripes) {
       const int64_t rb = st * br;
-      if (lane == 0) v = a.indptr[rb];
-      if (lane == 1) v = a.indptr[min<int64_t>(rb + br, a.rows)];
-      if (lane == 2) v = o.nt[st];
-      if (lane == 3) v = o.nv[st];
-      if (lane == 4) v = o.nd[st];
-      if (lane == 5) v = o.nd[st + 1];
-      if (lane == 6) v = o.nr[rb];
-      if (lane == 7) v = o.big[st];
+      hd.E0 = sload(a.indptr, rb);
+      hd.E1 = sload(a.indptr, min<int64_t>(rb + br, a.rows));
+      hd.tile = sload(o.nt, st);
+      hd.val = sload(o.nv, st);
+      hd.d0 = sload(o.nd, st);
+      hd.d1 = sload(o.nd, st + 1);
+      hd.rem = sload(o.nr, rb);
+      hd.big = sload(o.big, st + 1) - sload(o.big, st);
     }
-    return v;
+    return hd;
   };
-  int32_t head = load_head(wave);
+  Head head = load_head(wave);
   // the last wave closes the offset arrays: offsets[T] = V, stripe_ptr[ns] = T
   if (wave == nwaves - 1 && lane == 0) {
     const int32_t T = o.nt[a.num_stripes];
@@ -526,16 +537,13 @@ __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildA
 
   for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
     const int64_t r0 = s * br;
-    const int32_t E0 = __builtin_amdgcn_readlane(head, 0);
-    const int32_t E1 = __builtin_amdgcn_readlane(head, 1);
-    const int32_t tile_base = __builtin_amdgcn_readlane(head, 2);
-    const int32_t val_base = __builtin_amdgcn_readlane(head, 3);
-    const int32_t d_base = __builtin_amdgcn_readlane(head, 4);
-    const int32_t D = __builtin_amdgcn_readlane(head, 5) - d_base;
-    const int32_t rem_base = __builtin_amdgcn_readlane(head, 6);
-    const int32_t big = __builtin_amdgcn_readlane(head, 7);
+    const Head cur = head;
+    const int32_t E0 = cur.E0, E1 = cur.E1;
+    const int32_t tile_base = cur.tile, val_base = cur.val, d_base = cur.d0;
+    const int32_t D = cur.d1 - cur.d0;
+    const int32_t rem_base = cur.rem;
     if (lane == 0 && o.stripe_ptr) o.stripe_ptr[s] = tile_base;
-    if (big) {  // the merge kernel fills this stripe
+    if (cur.big) {  // the merge kernel fills this stripe
       head = load_head(s + nwaves);
       continue;
     }
@@ -574,7 +582,6 @@ __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildA
       if (valid) {
         h.key[j] = bkey[t];
         h.code[j] = bcode[t];
-        h.voff[j] = -1;
       }
       const uint64_t m = __ballot(listed);
       if (listed) {
@@ -612,6 +619,9 @@ __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildA
       for (int t = 0; t < kHashCap / kWave; ++t)
         if (t * kWave + lane < L) h.lidx[rank[t]] = (uint8_t)myj[t];
     }
+    wsync();
+    // the ranked keys are done with: their space holds the value offsets, -1 = rejected
+    for (int j = lane; j < D; j += kWave) h.voff[j] = -1;
     wsync();
 
     // ---- tile indices and value offsets by prefix sums in block-column order
@@ -781,31 +791,34 @@ int excl_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, hipStrea
   return GL_OK;
 }
 
-// ---- the builder's four scans (tiles, tile values and distinct blocks per stripe,
-// remainder entries per row) in two launches: tile-local scans of all four arrays, then one
-// that adds each tile's carry (the sum of the tile sums before it, read by the tile itself)
-// and writes out[n] and the four totals next to each other for a single copy to the host.
-// Arrays of more than kScan4MaxTiles tiles take the three-launch scan above instead.
-constexpr int kScan4MaxTiles = 2048;
-struct Scan4 {
-  int32_t* a[4];
-  int64_t n[4];
-  int32_t* bsum[4];
-  int32_t tiles[4];
-  int32_t first[5];   // first workgroup of array i in the tile launch
-  int32_t afirst[5];  // ... in the add launch (at least one workgroup per array)
-  int32_t* totals;
+// ---- the builder's scans (tiles, tile values and distinct blocks per stripe, remainder
+// entries per row, overflowing stripes) in two launches: tile-local scans of all arrays,
+// then one that adds each tile's carry (the sum of the tile sums before it, read by the tile
+// itself) and writes out[n] and the totals, to device memory and to pinned host memory, so
+// the host reads them after a stream synchronization without a copy.
+// Arrays of more than kScanMaxTiles tiles take the three-launch scan above instead.
+constexpr int kScanArrays = 5;
+constexpr int kScanMaxTiles = 2048;
+struct ScanSet {
+  int32_t* a[kScanArrays];
+  int64_t n[kScanArrays];
+  int32_t* bsum[kScanArrays];
+  int32_t tiles[kScanArrays];
+  int32_t first[kScanArrays + 1];   // first workgroup of array i in the tile launch
+  int32_t afirst[kScanArrays + 1];  // ... in the add launch (at least one per array)
+  int32_t* totals;                  // device
+  int32_t* host_totals;             // pinned host memory (device-visible), or NULL
 };
 
-__device__ __forceinline__ int scan4_array(const int32_t* first, int32_t blk) {
+__device__ __forceinline__ int scan_array(const int32_t* first, int32_t blk) {
   int i = 0;
-  while (i < 3 && blk >= first[i + 1]) ++i;
+  while (i < kScanArrays - 1 && blk >= first[i + 1]) ++i;
   return i;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan4_tiles(Scan4 p) {
+__global__ void __launch_bounds__(kScanThreads) k_scanset_tiles(ScanSet p) {
   __shared__ int32_t sh[65];
-  const int i = scan4_array(p.first, (int32_t)blockIdx.x);
+  const int i = scan_array(p.first, (int32_t)blockIdx.x);
   const int64_t j = blockIdx.x - p.first[i];
   int32_t* __restrict__ a = p.a[i];
   const int64_t n = p.n[i];
@@ -827,9 +840,17 @@ __global__ void __launch_bounds__(kScanThreads) k_scan4_tiles(Scan4 p) {
   if (threadIdx.x == 0) p.bsum[i][j] = total;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan4_add(Scan4 p) {
+__device__ __forceinline__ void put_total(const ScanSet& p, int i, int32_t total) {
+  p.totals[i] = total;
+  if (p.host_totals) {
+    p.host_totals[i] = total;
+    __threadfence_system();
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scanset_add(ScanSet p) {
   __shared__ int32_t sh[65];
-  const int i = scan4_array(p.afirst, (int32_t)blockIdx.x);
+  const int i = scan_array(p.afirst, (int32_t)blockIdx.x);
   const int64_t j = blockIdx.x - p.afirst[i];
   int32_t* __restrict__ a = p.a[i];
   const int64_t n = p.n[i];
@@ -849,53 +870,53 @@ __global__ void __launch_bounds__(kScanThreads) k_scan4_add(Scan4 p) {
   if (threadIdx.x == 0 && j + 1 >= p.tiles[i]) {
     const int32_t total = carry + (p.tiles[i] > 0 ? bs[j] : 0);
     a[n] = total;
-    p.totals[i] = total;
+    put_total(p, i, total);
   }
 }
 
-__global__ void k_scan4_totals(int32_t* a0, int64_t n0, int32_t* a1, int64_t n1, int32_t* a2,
-                               int64_t n2, int32_t* a3, int64_t n3, int32_t* totals) {
-  if (threadIdx.x == 0) {
-    totals[0] = a0[n0];
-    totals[1] = a1[n1];
-    totals[2] = a2[n2];
-    totals[3] = a3[n3];
-  }
+// totals of arrays scanned by the three-launch scan
+__global__ void k_scanset_totals(ScanSet p) {
+  if (threadIdx.x < kScanArrays) put_total(p, threadIdx.x, p.a[threadIdx.x][p.n[threadIdx.x]]);
 }
 
-// In-place exclusive scans of a[i][0..n[i]) with a[i][n[i]] = total, totals[i] = total.
-// bsum: kScan4MaxTiles ints per array, or scan_tmp_elems() for the three-launch scan.
-int scan4(int32_t* const a[4], const int64_t n[4], int32_t* bsum, int32_t* totals,
-          hipStream_t st) {
-  Scan4 p{};
+// In-place exclusive scans of a[i][0..n[i]) with a[i][n[i]] = total, totals[i] = total (and
+// host_totals[i] when not NULL). bsum: kScanMaxTiles ints per array, or scan_tmp_elems()
+// for the three-launch scan.
+int scan_set(int32_t* const a[kScanArrays], const int64_t n[kScanArrays], int32_t* bsum,
+             int32_t* totals, int32_t* host_totals, hipStream_t st) {
+  ScanSet p{};
   int64_t max_tiles = 0;
-  for (int i = 0; i < 4; ++i) max_tiles = std::max(max_tiles, (n[i] + kScanTile - 1) / kScanTile);
-  if (max_tiles > kScan4MaxTiles) {
-    for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < kScanArrays; ++i) {
+    p.a[i] = a[i];
+    p.n[i] = n[i];
+    max_tiles = std::max(max_tiles, (n[i] + kScanTile - 1) / kScanTile);
+  }
+  p.totals = totals;
+  p.host_totals = host_totals;
+  if (max_tiles > kScanMaxTiles) {
+    for (int i = 0; i < kScanArrays; ++i) {
       int rc = excl_scan(a[i], a[i], n[i], bsum, st);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_scan4_totals, dim3(1), dim3(64), 0, st, a[0], n[0], a[1], n[1], a[2],
-                       n[2], a[3], n[3], totals);
-    GL_LAUNCH_CHECK("k_scan4_totals");
+    hipLaunchKernelGGL(k_scanset_totals, dim3(1), dim3(64), 0, st, p);
+    GL_LAUNCH_CHECK("k_scanset_totals");
     return GL_OK;
   }
   p.first[0] = p.afirst[0] = 0;
-  for (int i = 0; i < 4; ++i) {
-    p.a[i] = a[i];
-    p.n[i] = n[i];
-    p.bsum[i] = bsum + (int64_t)i * kScan4MaxTiles;
+  for (int i = 0; i < kScanArrays; ++i) {
+    p.bsum[i] = bsum + (int64_t)i * kScanMaxTiles;
     p.tiles[i] = (int32_t)((n[i] + kScanTile - 1) / kScanTile);
     p.first[i + 1] = p.first[i] + p.tiles[i];
     p.afirst[i + 1] = p.afirst[i] + std::max(p.tiles[i], 1);
   }
-  p.totals = totals;
-  if (p.first[4] > 0) {
-    hipLaunchKernelGGL(k_scan4_tiles, dim3((unsigned)p.first[4]), dim3(kScanThreads), 0, st, p);
-    GL_LAUNCH_CHECK("k_scan4_tiles");
+  if (p.first[kScanArrays] > 0) {
+    hipLaunchKernelGGL(k_scanset_tiles, dim3((unsigned)p.first[kScanArrays]),
+                       dim3(kScanThreads), 0, st, p);
+    GL_LAUNCH_CHECK("k_scanset_tiles");
   }
-  hipLaunchKernelGGL(k_scan4_add, dim3((unsigned)p.afirst[4]), dim3(kScanThreads), 0, st, p);
-  GL_LAUNCH_CHECK("k_scan4_add");
+  hipLaunchKernelGGL(k_scanset_add, dim3((unsigned)p.afirst[kScanArrays]), dim3(kScanThreads),
+                     0, st, p);
+  GL_LAUNCH_CHECK("k_scanset_add");
   return GL_OK;
 }
 
@@ -905,7 +926,7 @@ int scan4(int32_t* const a[4], const int64_t n[4], int32_t* bsum, int32_t* total
 // first entry and each entry's block index and bit.
 struct BuildWs {
   int32_t *nt, *nv, *nd, *tmp, *tot;
-  uint8_t* big;
+  int32_t* big;
   uint64_t* wcode;
   int32_t* wkey;
   uint16_t* winfo;
@@ -915,7 +936,7 @@ static int64_t al(int64_t x) { return (x + 255) & ~(int64_t)255; }
 static BuildWs carve_build(char* base, int64_t rows, int32_t br, int64_t nnz) {
   const int64_t ns = (rows + br - 1) / br;
   const int64_t tmpn = std::max<int64_t>(scan_tmp_elems(std::max(rows, ns) + 1),
-                                         4 * (int64_t)kScan4MaxTiles);
+                                         kScanArrays * (int64_t)kScanMaxTiles);
   BuildWs w;
   int64_t off = 0;
   auto take = [&](int64_t n) {
@@ -927,8 +948,8 @@ static BuildWs carve_build(char* base, int64_t rows, int32_t br, int64_t nnz) {
   w.nv = (int32_t*)take((ns + 1) * 4);
   w.nd = (int32_t*)take((ns + 1) * 4);
   w.tmp = (int32_t*)take(tmpn * 4);
-  w.tot = (int32_t*)take(4 * 4);
-  w.big = (uint8_t*)take(ns);
+  w.tot = (int32_t*)take(kScanArrays * 4);
+  w.big = (int32_t*)take((ns + 1) * 4);
   w.wcode = (uint64_t*)take(nnz * 8);
   w.wkey = (int32_t*)take(nnz * 4);
   w.winfo = (uint16_t*)take(nnz * 2);
@@ -958,21 +979,22 @@ static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t 
   }
   const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
   const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
+  const unsigned groups = (unsigned)((nw + 3) / 4);
   if constexpr (FILL)
-    hipLaunchKernelGGL((k_hash_fill<V, P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st,
-                       a, o);
+    hipLaunchKernelGGL((k_hash_fill<V, P2>), dim3(groups), dim3(256), 0, st, a, o);
   else
-    hipLaunchKernelGGL((k_hash_count<P2>), dim3((unsigned)((nw + 3) / 4)), dim3(256), 0, st, a,
-                       o);
+    hipLaunchKernelGGL((k_hash_count<P2>), dim3(groups), dim3(256), 0, st, a, o);
 }
 
 template <bool FILL, typename V>
-static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st) {
+static int launch_build(const BuildArgs& a, const FillOut& o, hipStream_t st,
+                        bool merge = true) {
   if (o.big != nullptr) {
     if ((a.bc & (a.bc - 1)) == 0) launch_build_hash<FILL, V, true>(a, o, st);
     else launch_build_hash<FILL, V, false>(a, o, st);
     GL_LAUNCH_CHECK("k_hash_count / k_hash_fill");
   }
+  if (!merge) return GL_OK;
   const int GLn = group_lanes(a.br);
   const int spw = kWave / GLn;
   const int64_t waves = (a.num_stripes + spw - 1) / spw;
@@ -1092,21 +1114,24 @@ int gl_pbr_build_count(int64_t rows, int64_t cols, int64_t nnz, int32_t block_ro
   o.wkey = w.wkey;
   o.winfo = w.winfo;
   if (ns > 0 && (rc = launch_build<false, uint32_t>(a, o, st))) return rc;
-  int32_t* arrs[4] = {w.nt, w.nv, rem_indptr, w.nd};
-  const int64_t lens[4] = {ns, ns, rows, ns};
-  if ((rc = scan4(arrs, lens, w.tmp, w.tot, st))) return rc;
-  // one copy of the four totals into pinned memory of this thread
+  // the last scan kernel also writes the totals to pinned host memory of this thread
   static thread_local int32_t* h_tot = nullptr;
-  if (!h_tot) GL_HIP(hipHostMalloc((void**)&h_tot, 4 * sizeof(int32_t), hipHostMallocDefault));
-  GL_HIP(hipMemcpyAsync(h_tot, w.tot, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  static thread_local int32_t* d_tot = nullptr;
+  if (!h_tot) {
+    GL_HIP(hipHostMalloc((void**)&h_tot, kScanArrays * sizeof(int32_t), hipHostMallocCoherent));
+    GL_HIP(hipHostGetDevicePointer((void**)&d_tot, h_tot, 0));
+  }
+  int32_t* arrs[kScanArrays] = {w.nt, w.nv, rem_indptr, w.nd, w.big};
+  const int64_t lens[kScanArrays] = {ns, ns, rows, ns, ns};
+  if ((rc = scan_set(arrs, lens, w.tmp, w.tot, d_tot, st))) return rc;
   GL_HIP(hipStreamSynchronize(st));
-  for (int i = 0; i < 4; ++i) counts[i] = h_tot[i];
+  for (int i = 0; i < kScanArrays; ++i) counts[i] = h_tot[i];
   return GL_OK;
 }
 
 int gl_pbr_build_fill(int64_t rows, int64_t cols, int64_t nnz, int32_t block_rows,
                       int32_t block_cols, int32_t min_nnz_per_block, int64_t num_tiles,
-                      const int32_t* indptr, const int32_t* indices, const void* values,
+                      int64_t num_big_stripes, const int32_t* indptr, const int32_t* indices, const void* values,
                       int32_t value_bytes, void* workspace, int64_t workspace_bytes,
                       uint64_t* codes, int32_t* coords, int32_t* offsets, void* data,
                       const int32_t* rem_indptr, int32_t* rem_col_ind, void* rem_vals,
@@ -1147,9 +1172,11 @@ int gl_pbr_build_fill(int64_t rows, int64_t cols, int64_t nnz, int32_t block_row
   o.wkey = w.wkey;
   o.winfo = w.winfo;
   if (ns > 0) {
-    // the fill kernel also writes offsets[T] = V and stripe_ptr[ns] = T
-    rc = value_bytes == 4 ? launch_build<true, uint32_t>(a, o, st)
-                          : launch_build<true, uint64_t>(a, o, st);
+    // the fill kernel also writes offsets[T] = V and stripe_ptr[ns] = T; the merge kernel
+    // runs when the count pass left it stripes (or their number is not given)
+    const bool merge = num_big_stripes != 0;
+    rc = value_bytes == 4 ? launch_build<true, uint32_t>(a, o, st, merge)
+                          : launch_build<true, uint64_t>(a, o, st, merge);
     if (rc) return rc;
   } else {
     GL_HIP(hipMemcpyAsync(offsets + num_tiles, w.nv, 4, hipMemcpyDeviceToDevice, st));

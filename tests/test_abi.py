@@ -38,6 +38,26 @@ def test_ctypes_table_matches_header():
     assert sorted(_native.exported_symbols()) == declared_functions()
 
 
+def declared_arity():
+    """Number of parameters of every declared function (0 for `(void)`)."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    out = {}
+    for m in re.finditer(r"(gl_[A-Za-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_argument_counts_match_header():
+    """A missing or extra ctypes argument shifts every later one (a wrong pointer)."""
+    arity = declared_arity()
+    assert set(arity) == set(declared_functions())
+    wrong = {n: (len(args), arity[n]) for n, (_, args) in _native._PROTOS.items()
+             if len(args) != arity[n]}
+    assert not wrong, wrong
+
+
 def test_library_loads_and_reports_version():
     L = _native.lib()
     assert L.gl_abi_version() == 9
