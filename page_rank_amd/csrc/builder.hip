@@ -253,10 +253,8 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// kHashCap = 256 slots: bits 16..23 of the low 24 bits of b times a Fibonacci-hashing
-// constant (a full-rate 24-bit multiply)
 __device__ __forceinline__ int hash_slot(int32_t b) {
-  return (int)((__umul24((uint32_t)b, 0x9E3779u) >> 16) & 255u);
+  return (int)(((uint32_t)b * 2654435761u) >> 24);  // kHashCap = 256
 }
 // bits [0, n) of a 64-bit mask, n in [0, 64]
 __device__ __forceinline__ uint64_t low_bits(int n) {
@@ -281,9 +279,12 @@ __global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs
   const int lane = threadIdx.x & (kWave - 1);
   CountLds& h = sh_all[threadIdx.x / kWave];
   // uniform across the wave: scalar registers
-  const int64_t wave = __builtin_amdgcn_readfirstlane(
-      (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
-  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  // stripe indices in 32 bits (fewer scalar registers): stripes < 2^31, and unsigned sums of
+  // a stripe and two grid strides stay below 2^32
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  const uint32_t num_stripes = (uint32_t)a.num_stripes;
   const int br = a.br, bc = a.bc;
   const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
   const uint64_t below_lane = low_bits(lane);
@@ -292,10 +293,10 @@ __global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs
     return P2 ? ((r << bc_shift) | (col & (bc - 1))) : r * bc + (col - b * bc);
   };
   // CSR row starts of a stripe: row k in lane k, row 64 in lane 0's second register
-  auto load_rows = [&](int64_t st, int32_t& rp0, int32_t& rp1) {
+  auto load_rows = [&](uint32_t st, int32_t& rp0, int32_t& rp1) {
     rp0 = rp1 = 0;
-    if (st >= a.num_stripes) return;
-    const int64_t rb = st * br;
+    if (st >= num_stripes) return;
+    const int64_t rb = (int64_t)st * br;
     if (lane <= br) rp0 = a.indptr[min<int64_t>(rb + lane, a.rows)];
     if (lane == 0 && br >= kWave) rp1 = a.indptr[min<int64_t>(rb + kWave, a.rows)];
   };
@@ -319,8 +320,8 @@ __global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs
   load_cols(E0, E1, pcol);
   load_rows(wave + nwaves, nrp0, nrp1);
 
-  for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
-    const int64_t r0 = s * br;
+  for (uint32_t s = wave; s < num_stripes; s += nwaves) {
+    const int64_t r0 = (int64_t)s * br;
     if (lane <= br) h.rows[lane] = rp0;
     if (lane == 0 && br >= kWave) h.rows[kWave] = rp1;
     for (int i = lane; i < kHashCap; i += kWave) {
@@ -406,7 +407,7 @@ __global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs
     const int32_t cE0 = E0, cE1 = E1;
     rp0 = nrp0;
     rp1 = nrp1;
-    if (s + nwaves < a.num_stripes) {
+    if (s + nwaves < num_stripes) {
       E0 = __builtin_amdgcn_readfirstlane(rp0);
       E1 = end_of(rp0, rp1);
       load_cols(E0, E1, pcol);
@@ -492,30 +493,34 @@ __global__ void __launch_bounds__(256, kHashCountWgPerCu) k_hash_count(BuildArgs
 
 // 8 waves per SIMD (64 VGPRs), which is also what its LDS allows (8 workgroups per CU).
 constexpr int kHashFillWavesPerSimd = 8;
-template <typename V, bool P2>
+// ALL: the reference-order block list (every distinct block) is written too.
+template <typename V, bool P2, bool ALL>
 __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildArgs a, FillOut o) {
   __shared__ FillLds sh_all[4];
   const int lane = threadIdx.x & (kWave - 1);
   FillLds& h = sh_all[threadIdx.x / kWave];
-  const int64_t wave = __builtin_amdgcn_readfirstlane(
-      (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kWave));
-  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  // stripe indices in 32 bits (fewer scalar registers): stripes < 2^31, and unsigned sums of
+  // a stripe and two grid strides stay below 2^32
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(
+      (blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const uint32_t nwaves = gridDim.x * blockDim.x / kWave;
+  const uint32_t num_stripes = (uint32_t)a.num_stripes;
   const V* __restrict__ vals = (const V*)o.values;
   V* __restrict__ data = (V*)o.data;
   V* __restrict__ rem_val = (V*)o.rem_val;
   const int br = a.br, bc = a.bc;
   const int bc_shift = P2 ? __builtin_ctz((unsigned)bc) : 0;
   const uint64_t below_lane = low_bits(lane);
-  const bool all_blocks = o.blk_bc != nullptr;
+  constexpr bool all_blocks = ALL;
   // a stripe's scalars, scalar loads issued a stripe ahead: first and end entry, tile /
   // value / distinct-block offsets, next distinct-block offset, remainder offset, overflow
   struct Head {
     int32_t E0, E1, tile, val, d0, d1, rem, big;
   };
-  auto load_head = [&](int64_t st) {
+  auto load_head = [&](uint32_t st) {
     Head hd{0, 0, 0, 0, 0, 0, 0, 0};
-    if (st < a.num_stripes) {
-      const int64_t rb = st * br;
+    if (st < num_stripes) {
+      const int64_t rb = (int64_t)st * br;
       hd.E0 = sload(a.indptr, rb);
       hd.E1 = sload(a.indptr, min<int64_t>(rb + br, a.rows));
       hd.tile = sload(o.nt, st);
@@ -535,8 +540,8 @@ __global__ void __launch_bounds__(256, kHashFillWavesPerSimd) k_hash_fill(BuildA
     if (o.stripe_ptr) o.stripe_ptr[a.num_stripes] = T;
   }
 
-  for (int64_t s = wave; s < a.num_stripes; s += nwaves) {
-    const int64_t r0 = s * br;
+  for (uint32_t s = wave; s < num_stripes; s += nwaves) {
+    const int64_t r0 = (int64_t)s * br;
     const Head cur = head;
     const int32_t E0 = cur.E0, E1 = cur.E1;
     const int32_t tile_base = cur.tile, val_base = cur.val, d_base = cur.d0;
@@ -971,7 +976,7 @@ static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t 
   if (per_cu.load() < 0) {
     int n = kHashCountWgPerCu;
     if (FILL &&
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hash_fill<V, P2>, 256, 0) !=
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_hash_fill<V, P2, false>, 256, 0) !=
              hipSuccess ||
          n < 1))
       n = 4;
@@ -980,8 +985,12 @@ static void launch_build_hash(const BuildArgs& a, const FillOut& o, hipStream_t 
   const int64_t cap = (int64_t)device_info().num_cus * per_cu.load() * 4;
   const int64_t nw = std::max<int64_t>(4, std::min(a.num_stripes, cap));
   const unsigned groups = (unsigned)((nw + 3) / 4);
-  if constexpr (FILL)
-    hipLaunchKernelGGL((k_hash_fill<V, P2>), dim3(groups), dim3(256), 0, st, a, o);
+  if constexpr (FILL) {
+    if (o.blk_bc != nullptr)
+      hipLaunchKernelGGL((k_hash_fill<V, P2, true>), dim3(groups), dim3(256), 0, st, a, o);
+    else
+      hipLaunchKernelGGL((k_hash_fill<V, P2, false>), dim3(groups), dim3(256), 0, st, a, o);
+  }
   else
     hipLaunchKernelGGL((k_hash_count<P2>), dim3(groups), dim3(256), 0, st, a, o);
 }
