@@ -129,19 +129,21 @@ def _build(shape, nnz, indptr, indices, vals, it, st, block_rows, block_cols,
             outs = [buf[o:o + n * dt.itemsize].view(dt) for o, (n, dt) in zip(starts, shapes)]
         codes, coords, offsets, data, rem_col, rem_vals, stripe_ptr = outs
         if want_blocks:
-            h_ptr = blk_ptr.cpu().numpy()
-            h_bc = blk_bc.cpu().numpy()
-            h_rmin = blk_rmin.cpu().numpy()
-            h_tile = blk_tile.cpu().numpy()
-            perm = np.empty(T, dtype=np.int32)
+            # block lists to pinned host memory (one stream synchronization), the replay of
+            # the reference's container on the host, the permutation back the same way
+            hosts = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                     for t in (blk_ptr, blk_bc, blk_rmin, blk_tile)]
+            for h, t in zip(hosts, (blk_ptr, blk_bc, blk_rmin, blk_tile)):
+                h.copy_(t, non_blocking=True)
+            torch.cuda.current_stream(dev).synchronize()
+            h_ptr, h_bc, h_rmin, h_tile = (h.numpy() for h in hosts)
+            perm_h = torch.empty(T, dtype=torch.int32, pin_memory=True)
             _native.check(L.gl_pbr_reference_perm(rows, cols, block_rows, block_cols,
                                                   it.itemsize, h_ptr.ctypes.data,
                                                   h_bc.ctypes.data, h_rmin.ctypes.data,
-                                                  h_tile.ctypes.data, perm.ctypes.data),
+                                                  h_tile.ctypes.data, perm_h.data_ptr()),
                           "gl_pbr_reference_perm")
-            if not np.array_equal(np.sort(perm), np.arange(T, dtype=np.int32)):
-                raise RuntimeError("reference-order replay did not produce a permutation")
-            perm_d = torch.as_tensor(perm).to(dev)
+            perm_d = perm_h.to(dev, non_blocking=True)
             codes2 = torch.empty_like(codes)
             coords2 = torch.empty_like(coords)
             offsets2 = torch.empty_like(offsets)

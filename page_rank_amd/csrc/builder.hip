@@ -19,6 +19,7 @@
 #include <atomic>
 #include <type_traits>
 #include <cmath>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1044,31 +1045,162 @@ __global__ void k_tile_gather(const int32_t* __restrict__ perm, int64_t T,
   for (int32_t i = 0; i < n; ++i) data_o[d + i] = data[a + i];
 }
 
+// Reference emission order without the container: the reference builder keeps one
+// std::unordered_map<index_t, ...>(sqrt(num_col_blocks)), emplaces each stripe's blocks in
+// CSR first-occurrence order, iterates the map and clear()s it (pbr_matrix.cpp:374-428).
+// libstdc++'s _Hashtable (identity hash on the key converted to size_t, unique keys) gives
+// that iteration a fixed order: a node entering an empty bucket goes to the front of the
+// one list and becomes its bucket's "before" node for the bucket that was first; a node
+// entering an occupied bucket goes right after that bucket's "before" node; a rehash relinks
+// the list front to back the same way (hashtable.h, _M_insert_bucket_begin and
+// _M_rehash_aux). That order is replayed here with arrays, per stripe on worker threads.
+// Only the bucket count needs the library: it depends on the number of elements alone, and
+// it can change only in a stripe holding more blocks than every stripe before it (clear()
+// keeps the buckets and the rehash threshold), so one real map of the same type, fed
+// placeholder keys for those stripes only, gives the bucket count before every insert.
+// a mod d for a fixed d without a division (Lemire, Kaser and Kurz, "Faster Remainder by
+// Direct Computation", 2019): M = floor((2^128 - 1) / d) + 1, a mod d = high 64 bits of
+// ((M * a) mod 2^128) * d.
+struct FastMod64 {
+  uint64_t d = 1;
+  unsigned __int128 m = 0;
+  void set(uint64_t div) {
+    d = div;
+    m = ~(unsigned __int128)0 / div + 1;
+  }
+  uint64_t operator()(uint64_t a) const {
+    const unsigned __int128 low = m * a;
+    const unsigned __int128 bottom = ((unsigned __int128)(uint64_t)low * d) >> 64;
+    const unsigned __int128 top = (unsigned __int128)(uint64_t)(low >> 64) * d;
+    return (uint64_t)((bottom + top) >> 64);
+  }
+};
+
+// Returns false when a stripe's kept blocks do not carry consecutive tile indices.
 template <typename I>
-static void replay(int64_t ns, int64_t cols, int32_t br, int32_t bc, const int32_t* blk_ptr,
+static bool replay(int64_t ns, int64_t cols, int32_t br, int32_t bc, const int32_t* blk_ptr,
                    const int32_t* blk_bc, const int8_t* blk_rmin, const int32_t* blk_tile,
                    int32_t* perm) {
   const uint64_t ncb = ((uint64_t)cols + bc - 1) / bc;
-  std::unordered_map<I, int32_t> m((size_t)std::sqrt((double)ncb));
-  std::vector<int32_t> order;
-  std::vector<int32_t> bucket_start(br + 1);
-  int64_t out = 0;
-  for (int64_t s = 0; s < ns; ++s) {
-    const int32_t d0 = blk_ptr[s], d1 = blk_ptr[s + 1];
-    // insertion order of the reference: CSR order of first occurrence = (first row, block col)
-    std::fill(bucket_start.begin(), bucket_start.end(), 0);
-    for (int32_t d = d0; d < d1; ++d) bucket_start[blk_rmin[d] + 1]++;
-    for (int i = 0; i < br; ++i) bucket_start[i + 1] += bucket_start[i];
-    order.assign(d1 - d0, 0);
-    for (int32_t d = d0; d < d1; ++d) order[bucket_start[blk_rmin[d]]++] = d;
-    for (int32_t d : order) {
-      const uint64_t block_index = (uint64_t)s * ncb + (uint64_t)blk_bc[d];
-      m.emplace((I)block_index, d);
+  if (ns <= 0) return true;
+  // bucket counts: bstart[s] before stripe s; a stripe with more blocks than any before it
+  // (rec_off[s] >= 0) takes rec_bc[rec_off[s] + i] at its i-th insert
+  std::vector<uint64_t> bstart(ns);
+  std::vector<int64_t> rec_off(ns, -1);
+  std::vector<uint64_t> rec_bc;
+  int32_t maxd = 0;
+  {
+    std::unordered_map<I, int32_t> probe((size_t)std::sqrt((double)ncb));
+    uint64_t cur = probe.bucket_count();
+    for (int64_t st = 0; st < ns; ++st) {
+      bstart[st] = cur;
+      const int32_t D = blk_ptr[st + 1] - blk_ptr[st];
+      if (D > maxd) {
+        rec_off[st] = (int64_t)rec_bc.size();
+        probe.clear();
+        for (int32_t i = 0; i < D; ++i) {
+          probe.emplace((I)i, 0);
+          rec_bc.push_back(probe.bucket_count());
+        }
+        maxd = D;
+        cur = probe.bucket_count();
+      }
     }
-    for (const auto& kv : m)
-      if (blk_tile[kv.second] >= 0) perm[out++] = blk_tile[kv.second];
-    m.clear();
   }
+  const uint64_t nbkt_max = rec_bc.empty() ? bstart[0] : std::max(bstart[0], rec_bc.back());
+
+  std::atomic<bool> ok{true};
+  auto run = [&](int64_t s_begin, int64_t s_end) {
+    constexpr int32_t kNil = -1, kFront = -2;  // end of list; the list's front ("before begin")
+    std::vector<int32_t> nxt(maxd), ord(maxd), before(nbkt_max), stamp(nbkt_max, 0);
+    std::vector<uint64_t> hv(maxd);
+    std::vector<int32_t> cnt(br + 1);
+    int32_t gen = 0;
+    FastMod64 mod;
+    for (int64_t st = s_begin; st < s_end; ++st) {
+      const int32_t d0 = blk_ptr[st], D = blk_ptr[st + 1] - d0;
+      if (D == 0) continue;
+      // insertion order: CSR order of first occurrence = (first row, block column)
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (int32_t j = 0; j < D; ++j) cnt[blk_rmin[d0 + j] + 1]++;
+      for (int r = 0; r < br; ++r) cnt[r + 1] += cnt[r];
+      for (int32_t j = 0; j < D; ++j) ord[cnt[blk_rmin[d0 + j]]++] = j;
+      uint64_t nb = bstart[st];
+      if (mod.d != nb) mod.set(nb);
+      int32_t head = kNil;
+      ++gen;
+      auto after = [&](int32_t prev) { return prev == kFront ? head : nxt[prev]; };
+      auto link_after = [&](int32_t prev, int32_t node) {
+        if (prev == kFront) head = node;
+        else nxt[prev] = node;
+      };
+      for (int32_t i = 0; i < D; ++i) {
+        const int32_t node = ord[i];
+        const uint64_t key = (uint64_t)st * ncb + (uint64_t)blk_bc[d0 + node];
+        hv[node] = (uint64_t)(int64_t)(I)key;  // the key as index_t, then as size_t
+        const uint64_t want = rec_off[st] >= 0 ? rec_bc[rec_off[st] + i] : nb;
+        if (want != nb) {  // rehash: relink the list front to back into new buckets
+          nb = want;
+          mod.set(nb);
+          ++gen;
+          int32_t p = head, bfirst = 0;
+          head = kNil;
+          while (p != kNil) {
+            const int32_t nx = nxt[p];
+            const int32_t b = (int32_t)mod(hv[p]);
+            if (stamp[b] != gen) {
+              nxt[p] = head;
+              head = p;
+              stamp[b] = gen;
+              before[b] = kFront;
+              if (nxt[p] != kNil) before[bfirst] = p;
+              bfirst = b;
+            } else {
+              const int32_t prev = before[b];
+              nxt[p] = after(prev);
+              link_after(prev, p);
+            }
+            p = nx;
+          }
+        }
+        const int32_t b = (int32_t)mod(hv[node]);
+        if (stamp[b] == gen) {
+          const int32_t prev = before[b];
+          nxt[node] = after(prev);
+          link_after(prev, node);
+        } else {
+          nxt[node] = head;
+          head = node;
+          if (nxt[node] != kNil) before[(int32_t)mod(hv[nxt[node]])] = node;
+          stamp[b] = gen;
+          before[b] = kFront;
+        }
+      }
+      // emission: kept blocks in list order, at this stripe's first tile index (the kept
+      // blocks, in block-column order, carry consecutive tile indices from there)
+      int32_t out = -1, kept = 0;
+      for (int32_t j = 0; j < D; ++j) {
+        const int32_t t = blk_tile[d0 + j];
+        if (t < 0) continue;
+        if (out < 0) out = t;
+        if (t != out + kept) ok = false;
+        ++kept;
+      }
+      for (int32_t p = head; p != kNil && out >= 0; p = nxt[p])
+        if (blk_tile[d0 + p] >= 0) perm[out++] = blk_tile[d0 + p];
+    }
+  };
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int64_t nt = std::min<int64_t>({hw, 16, (ns + 4095) / 4096});
+  if (nt <= 1) {
+    run(0, ns);
+  } else {
+    std::vector<std::thread> pool;
+    for (int64_t t = 0; t < nt; ++t)
+      pool.emplace_back(run, ns * t / nt, ns * (t + 1) / nt);
+    for (auto& th : pool) th.join();
+  }
+  return ok.load();
 }
 
 }  // namespace gl
@@ -1204,10 +1336,13 @@ int gl_pbr_reference_perm(int64_t rows, int64_t cols, int32_t block_rows, int32_
   GL_REQUIRE(index_bytes == 4 || index_bytes == 8, "index_bytes must be 4 or 8");
   GL_REQUIRE(block_rows >= 1 && block_cols >= 1, "bad block size");
   const int64_t ns = (rows + block_rows - 1) / block_rows;
-  if (index_bytes == 4)
-    replay<int32_t>(ns, cols, block_rows, block_cols, blk_ptr, blk_bc, blk_rmin, blk_tile, perm);
-  else
-    replay<int64_t>(ns, cols, block_rows, block_cols, blk_ptr, blk_bc, blk_rmin, blk_tile, perm);
+  const bool ok =
+      index_bytes == 4
+          ? replay<int32_t>(ns, cols, block_rows, block_cols, blk_ptr, blk_bc, blk_rmin, blk_tile,
+                            perm)
+          : replay<int64_t>(ns, cols, block_rows, block_cols, blk_ptr, blk_bc, blk_rmin, blk_tile,
+                            perm);
+  GL_REQUIRE(ok, "block list: a stripe's kept blocks do not carry consecutive tile indices");
   return GL_OK;
 }
 
