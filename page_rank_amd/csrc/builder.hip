@@ -1256,12 +1256,13 @@ int gl_pbr_build_count(int64_t rows, int64_t cols, int64_t nnz, int32_t block_ro
   o.winfo = w.winfo;
   if (ns > 0 && (rc = launch_build<false, uint32_t>(a, o, st))) return rc;
   // the last scan kernel also writes the totals to pinned host memory of this thread
+  // (coherent and portable: any device of the process may be the current one)
   static thread_local int32_t* h_tot = nullptr;
-  static thread_local int32_t* d_tot = nullptr;
-  if (!h_tot) {
-    GL_HIP(hipHostMalloc((void**)&h_tot, kScanArrays * sizeof(int32_t), hipHostMallocCoherent));
-    GL_HIP(hipHostGetDevicePointer((void**)&d_tot, h_tot, 0));
-  }
+  if (!h_tot)
+    GL_HIP(hipHostMalloc((void**)&h_tot, kScanArrays * sizeof(int32_t),
+                         hipHostMallocCoherent | hipHostMallocPortable));
+  int32_t* d_tot = nullptr;
+  GL_HIP(hipHostGetDevicePointer((void**)&d_tot, h_tot, 0));
   int32_t* arrs[kScanArrays] = {w.nt, w.nv, rem_indptr, w.nd, w.big};
   const int64_t lens[kScanArrays] = {ns, ns, rows, ns, ns};
   if ((rc = scan_set(arrs, lens, w.tmp, w.tot, d_tot, st))) return rc;
