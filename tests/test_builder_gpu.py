@@ -116,3 +116,57 @@ def test_noncanonical_input_is_canonicalized(device):
 def test_builder_type_errors():
     with pytest.raises(TypeError):
         graph_link.csr_to_pbr(np.zeros((3, 3)))
+
+
+@pytest.mark.parametrize("k", [200, 600])
+def test_builder_entry_points_report_overflowing_stripes(k, device):
+    """gl_pbr_build_count reports the stripes left to the merge kernel (counts[4]); the fill
+    pass given that number, or -1 (unknown), builds the same arrays as csr_to_pbr."""
+    import torch
+    from page_rank_amd import _native
+    n = 8 * 400
+    rng = np.random.default_rng(k)
+    r = np.concatenate([np.full(k, 2), rng.integers(8, n, 5000)])
+    c = np.concatenate([np.arange(k) * 8 + 1, rng.integers(0, n, 5000)]) % n
+    key = np.unique(r.astype(np.int64) * n + c)
+    A = sp.csr_matrix((rng.random(key.size).astype(np.float32), (key // n, key % n)),
+                      shape=(n, n))
+    want = graph_link.csr_to_pbr(A, 8, 8, 1, order="canonical")
+    L = _native.lib()
+    st = _native.stream_handle(device)
+    indptr = torch.as_tensor(A.indptr.astype(np.int32)).to(device)
+    indices = torch.as_tensor(A.indices.astype(np.int32)).to(device)
+    vals = torch.as_tensor(A.data).to(device)
+    wsb = L.gl_pbr_build_workspace_bytes(n, 8, A.nnz)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=device)
+    rem_indptr = torch.empty(n + 1, dtype=torch.int32, device=device)
+    counts = np.zeros(5, dtype=np.int64)
+    _native.check(L.gl_pbr_build_count(n, n, A.nnz, 8, 8, 1, indptr.data_ptr(),
+                                       indices.data_ptr(), rem_indptr.data_ptr(), ws.data_ptr(),
+                                       wsb, counts.ctypes.data, st), "count")
+    T, V, R, D, B = (int(x) for x in counts)
+    assert B == (1 if k > 256 else 0)
+    assert (T, V, R) == (want.accounted_blocks(), want.compressed_nnz(), want.remainder_nnz())
+    for nb in (B, -1):
+        out = [torch.empty(T, dtype=torch.int64, device=device),
+               torch.empty(2 * T, dtype=torch.int32, device=device),
+               torch.empty(T + 1, dtype=torch.int32, device=device),
+               torch.empty(V, dtype=torch.float32, device=device),
+               torch.empty(R, dtype=torch.int32, device=device),
+               torch.empty(R, dtype=torch.float32, device=device),
+               torch.empty(n // 8 + 1, dtype=torch.int32, device=device)]
+        p = [t.data_ptr() for t in out]
+        _native.check(L.gl_pbr_build_fill(
+            n, n, A.nnz, 8, 8, 1, T, nb, indptr.data_ptr(), indices.data_ptr(),
+            vals.data_ptr(), 4, ws.data_ptr(), wsb, p[0], p[1], p[2], p[3],
+            rem_indptr.data_ptr(), p[4], p[5], p[6], None, None, None, None, st), "fill")
+        torch.cuda.synchronize()
+        assert np.array_equal(out[0].cpu().numpy().view(np.uint64),
+                              np.asarray(want.block_codes).view(np.uint64))
+        assert np.array_equal(out[1].cpu().numpy(), np.asarray(want.block_coords))
+        assert np.array_equal(out[2].cpu().numpy(), np.asarray(want.block_offsets))
+        assert np.array_equal(out[3].cpu().numpy(), np.asarray(want.block_data))
+        assert np.array_equal(out[4].cpu().numpy(), np.asarray(want.remainder_col_ind))
+        assert np.array_equal(out[5].cpu().numpy(), np.asarray(want.remainder_data))
+        assert np.array_equal(rem_indptr.cpu().numpy(), np.asarray(want.remainder_indptr))
+        assert torch.equal(out[6], graph_link.get_pbr_gpu_meta(want)["stripe_ptr"])
