@@ -317,7 +317,7 @@ def ppr_leg(args, dev):
                        "iterations_mean": round(float(res6.iterations.mean()), 2)}
     log(f"  plain tol 1e-6: {K / t6:.1f} PPR/s, iterations max {res6.total_iterations}")
     # dominant-mode extrapolation every EXTRAP passes (run_ppr extrapolate_every)
-    EXTRAP = 6
+    EXTRAP = 5
     te, eres = _timed_runs(lambda: gl.run_ppr(pbr, sets, tol=tol_acc, extrapolate_every=EXTRAP,
                                               **kw))
     out["extrapolated"] = {"tol": tol_acc, "extrapolate_every": EXTRAP,
