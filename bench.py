@@ -456,17 +456,38 @@ def config1_leg(dev, reps=7):
            "cpu_absolute_tol": {"iterations": int(it_abs[0]), "converged": bool(conv_abs[0]),
                                 "l1_vs_exact": float(
                                     f"{np.abs(x_abs[:, 0] - exact).sum():.3e}")}}
-    for tol in (1e-8, 1.5e-7):
+    # the library's choice (the multi-workgroup loop at this size), then the whole run in
+    # one workgroup on the same graph
+    for name, tol, one in (("hip_tol_1e-08", 1e-8, None), ("hip_tol_1.5e-07", 1.5e-7, None),
+                           ("hip_one_wg_tol_1e-08", 1e-8, True)):
         tg, r = _timed_runs(lambda: gl.run_ppr(pbr, [[seed]], tol=tol, max_iter=100,
-                                               dangling=dang, dangling_mode="personalization"),
+                                               dangling=dang, dangling_mode="personalization",
+                                               single_workgroup=one),
                             reps=5)
         x = r.X[:, 0].cpu().numpy().astype(np.float64)
-        out[f"hip_tol_{tol:g}"] = {"ms": round(tg * 1e3, 3), "iterations": r.total_iterations,
-                                   "converged": bool(r.converged[0]),
-                                   "l1_vs_fp64": float(f"{np.abs(x - exact).sum():.3e}"),
-                                   "max_abs_vs_fp64": float(f"{np.abs(x - exact).max():.3e}")}
+        out[name] = {"ms": round(tg * 1e3, 3), "iterations": r.total_iterations,
+                     "converged": bool(r.converged[0]),
+                     "l1_vs_fp64": float(f"{np.abs(x - exact).sum():.3e}"),
+                     "max_abs_vs_fp64": float(f"{np.abs(x - exact).max():.3e}")}
     log(f"  config 1: CPU networkx path {t_nx*1e3:.2f} ms ({it_nx} it), HIP "
-        f"{out['hip_tol_1.5e-07']}")
+        f"{out['hip_tol_1.5e-07']}, one workgroup {out['hip_one_wg_tol_1e-08']}")
+    # request-sized graphs (the /api/ppr case): both paths, one seed, tol 1e-8
+    small = {}
+    for name, (ns, comm) in (("sbm_1k", (1000, 100)), ("sbm_100", (100, 25))):
+        s_src, s_dst = graphs.sbm_edges(n=ns, community=comm, p_in=0.04 * 200 / comm,
+                                        p_out=0.0002, seed=seed)
+        A_s, dang_s = graphs.transition_matrix(s_src, s_dst, ns)
+        pbr_s = gl.csr_to_pbr(A_s, 8, 8, 4, order="canonical")
+        row = {"nodes": ns, "nnz": int(A_s.nnz)}
+        for leg, one in (("one_wg", True), ("multi_wg", False)):
+            tg, r = _timed_runs(lambda: gl.run_ppr(pbr_s, [[0]], tol=1e-8, max_iter=100,
+                                                   dangling=dang_s,
+                                                   dangling_mode="personalization",
+                                                   single_workgroup=one), reps=7)
+            row[leg] = {"ms": round(tg * 1e3, 3), "iterations": r.total_iterations}
+        small[name] = row
+    out["request_sized"] = small
+    log(f"  request-sized graphs: {small}")
     return out
 
 

@@ -165,6 +165,12 @@ typedef struct gl_ppr_opts {
                           passes P, 2P, ... each active column whose residual ratio rho =
                           r_t / r_{t-1} lies in [0.5, 0.97] is extrapolated,
                           X += rho / (1 - rho) (X_t - X_{t-1}) (at most 4x), on the device */
+  int32_t single_wg;   /* 0 = library choice: the whole run in one launch of one workgroup
+                          when K <= 64 and (num_vals + num_rem) * K <= 2^13 (small graphs,
+                          where a pass of the multi-workgroup loop is bound by its launches);
+                          1 = always (K <= 64); -1 = never. Same step, residual, retire and
+                          extrapolation rules either way; graph_iters, kw and grid_mult apply
+                          to the multi-workgroup loop only */
 } gl_ppr_opts;
 
 int64_t gl_ppr_workspace_bytes(int64_t rows, int32_t block_rows, int64_t K,

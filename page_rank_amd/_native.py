@@ -42,7 +42,8 @@ class NativeError(RuntimeError):
 class GlOpts(ctypes.Structure):
     _fields_ = [("alpha", c_f32), ("tol", c_f32), ("max_iter", c_i32),
                 ("dangling_mode", c_i32), ("retire_mode", c_i32), ("graph_iters", c_i32),
-                ("kw", c_i32), ("grid_mult", c_i32), ("extrapolate_every", c_i32)]
+                ("kw", c_i32), ("grid_mult", c_i32), ("extrapolate_every", c_i32),
+                ("single_wg", c_i32)]
 
 
 # name -> (restype, argtypes)

@@ -183,7 +183,7 @@ extern "C" {
 
 const char* gl_last_error(void) { return t_last_error.c_str(); }
 
-int32_t gl_abi_version(void) { return 9; }
+int32_t gl_abi_version(void) { return 10; }
 
 int gl_device_info(int32_t* num_cus, int32_t* lds_bytes_per_cu, int32_t* clock_khz,
                    int64_t* hbm_bytes) {

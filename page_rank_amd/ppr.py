@@ -73,7 +73,7 @@ def _seed_arrays(seed_sets, weights=None):
 def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=None,
             dangling_mode="none", retire="per_column", x0=None, weights=None,
             graph_iters=8, kw=0, grid_mult=0, extrapolate_every=0,
-            return_history=True, x0_in_place=False) -> PPRResult:
+            return_history=True, x0_in_place=False, single_workgroup=None) -> PPRResult:
     """Batched PPR, one column per seed set (a list of node-index lists).
     kw / grid_mult: RHS columns per launch and workgroups per CU (0 = library rule;
     kw="auto" = measured by the autotuner, cached per shape).
@@ -82,7 +82,9 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
     X += rho / (1 - rho) (X_t - X_{t-1}) (Aitken extrapolation of the dominant error mode).
     The stopping rule is unchanged, so the result meets the same residual bound.
     x0_in_place: x0 (a contiguous float32 [N][K] tensor on the matrix's device) becomes the
-    result's buffer, with no copy; the caller's start is overwritten."""
+    result's buffer, with no copy; the caller's start is overwritten.
+    single_workgroup: None = library choice (the whole run in one launch of one workgroup for
+    small graphs: K <= 64 and nnz * K <= 2^13), True = always (K <= 64), False = never."""
     meta = get_pbr_gpu_meta(pbr_mat)
     if meta is None:
         raise RuntimeError("PBRMatrix is not on GPU. Call .to('cuda') first.")
@@ -133,7 +135,9 @@ def run_ppr(pbr_mat, seed_sets, *, alpha=0.85, tol=1e-6, max_iter=100, dangling=
     ws = torch.empty(wsb, dtype=torch.uint8, device=device)
     opts = _native.GlOpts(alpha=alpha, tol=tol, max_iter=int(max_iter), dangling_mode=dmode,
                           retire_mode=_RETIRE[retire], graph_iters=int(graph_iters), kw=int(kw),
-                          grid_mult=int(grid_mult), extrapolate_every=int(extrapolate_every))
+                          grid_mult=int(grid_mult), extrapolate_every=int(extrapolate_every),
+                          single_wg=0 if single_workgroup is None else
+                          (1 if single_workgroup else -1))
     total = np.zeros(1, dtype=np.int32)
     with torch.cuda.device(device):
         rc = L.gl_ppr_run_i32_f32(

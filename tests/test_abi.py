@@ -60,7 +60,7 @@ def test_ctypes_argument_counts_match_header():
 
 def test_library_loads_and_reports_version():
     L = _native.lib()
-    assert L.gl_abi_version() == 9
+    assert L.gl_abi_version() == 10
     assert L.gl_last_error() is not None
 
 

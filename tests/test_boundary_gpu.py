@@ -194,8 +194,9 @@ def test_concurrent_callers(device):
     pbr = graph_link.csr_to_pbr(A32, 8, 8, 4).to(device)
     sets = [[1], [2, 3], [400, 500, 600], [19999]]
     X = torch.rand((n, 24), generator=torch.Generator().manual_seed(1)).to(device)
+    # single_workgroup=False: the graph-captured multi-workgroup loop, whatever the size
     ref_ppr = graph_link.run_ppr(pbr, sets, tol=1e-6, max_iter=200, dangling=dang,
-                                 dangling_mode="uniform").X.cpu()
+                                 dangling_mode="uniform", single_workgroup=False).X.cpu()
     ref_y = graph_link.pbr_matmul(pbr, X, mode="split").cpu()
     fails = []
 
@@ -206,7 +207,8 @@ def test_concurrent_callers(device):
                 with torch.cuda.stream(s):
                     for _ in range(6):
                         r = graph_link.run_ppr(pbr, sets, tol=1e-6, max_iter=200,
-                                               dangling=dang, dangling_mode="uniform")
+                                               dangling=dang, dangling_mode="uniform",
+                                               single_workgroup=False)
                         s.synchronize()
                         if not torch.equal(r.X.cpu(), ref_ppr):
                             fails.append("ppr")

@@ -16,6 +16,20 @@ from oracle import ppr as oppr
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=[False, None], ids=["multi_wg", "auto"])
+def ppr_path(request, monkeypatch):
+    """Every test of this module runs twice: on the multi-workgroup loop (single_workgroup=
+    False) and with the library's choice, which runs small graphs in one workgroup."""
+    run = graph_link.run_ppr
+
+    def wrapped(*args, **kw):
+        kw.setdefault("single_workgroup", request.param)
+        return run(*args, **kw)
+
+    monkeypatch.setattr(graph_link, "run_ppr", wrapped)
+    return request.param
+
+
 def _build_graph_and_csr(N, p, seed=42):
     """Tests/test_ppr_accuracy.py:14-25."""
     G = nx.erdos_renyi_graph(N, p, directed=True, seed=seed)
