@@ -280,7 +280,8 @@ __global__ void __launch_bounds__(256) k_extrapolate(const PprCtl* __restrict__ 
 // S = kSingleThreads / KC, sl = t / KC) exactly as the step epilogue does:
 //   X_{t+1}[i,k] = alpha (A X_t)[i,k] + cu[k] + w[i,k] tele[k]   (frozen columns copied)
 // with the L1 residual and the dangling mass of its elements. Per column, the threads' sums
-// are combined in double by a fixed LDS tree; then the retire rule of k_retire (tolerance,
+// are combined in double in a fixed order (within each wave, then across the 16 waves); then
+// the retire rule of k_retire (tolerance,
 // together or per column, max_iter, extrapolation factor, dangling coefficients) runs on the
 // column state, which lives in LDS for the whole launch, and an extrapolation pass follows
 // when a column was given a factor. X_t and X_{t+1} are global buffers that only this
@@ -297,8 +298,8 @@ k_ppr_single(PbrView m, int32_t B, int64_t K, float* x0buf, float* x1buf, ColSta
              const int32_t* __restrict__ scol, const float* __restrict__ sw,
              const uint8_t* __restrict__ dmask, float* __restrict__ hist, float tol, float alpha,
              int dmode, int together, int max_iter, int extrap_every) {
-  __shared__ double s_r[kSingleThreads];
-  __shared__ double s_d[kSingleThreads];
+  __shared__ double s_r[kSingleThreads / 64 * kSingleMaxK];
+  __shared__ double s_d[kSingleThreads / 64 * kSingleMaxK];
   __shared__ float s_cu[kSingleMaxK], s_tele[kSingleMaxK], s_dsum[kSingleMaxK];
   __shared__ float s_res[kSingleMaxK], s_rprev[kSingleMaxK], s_beta[kSingleMaxK];
   __shared__ int s_frozen[kSingleMaxK];
@@ -365,23 +366,34 @@ k_ppr_single(PbrView m, int32_t B, int64_t K, float* x0buf, float* x1buf, ColSta
         xn[i * K + k] = xv;
       }
     }
-    s_r[t] = res;
-    s_d[t] = dang;
-    __syncthreads();
-    for (int h = S / 2; h >= 1; h /= 2) {
-      if (sl < h) {
-        s_r[t] += s_r[t + h * KC];
-        s_d[t] += s_d[t + h * KC];
+    // per column: a butterfly over the lanes of a wave that hold the column (lane distances
+    // 32, 16, ..., KC; both partners add the same two values, so every lane ends with the
+    // same bits), then the 16 wave sums in wave order
+    {
+      double r = res, d = dang;
+      for (int off = 32; off >= KC; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        d += __shfl_xor(d, off, 64);
       }
-      __syncthreads();
+      const int lane = t & 63;
+      if (lane < KC) {
+        s_r[(t >> 6) * KC + lane] = r;
+        s_d[(t >> 6) * KC + lane] = d;
+      }
     }
+    __syncthreads();
     // per column (the same rule as k_retire)
     float dsum_prev = 0.f;
     if (t < K && !s_frozen[t]) {
-      const float rf = (float)s_r[t];
+      double r = 0.0, d = 0.0;
+      for (int wv = 0; wv < kSingleThreads / 64; ++wv) {
+        r += s_r[wv * KC + t];
+        d += s_d[wv * KC + t];
+      }
+      const float rf = (float)r;
       s_res[t] = rf;
       dsum_prev = s_dsum[t];
-      s_dsum[t] = (float)s_d[t];
+      s_dsum[t] = (float)d;
       atomicMax(&s_max_bits, __float_as_int(rf));  // rf >= 0: int order == float order
     }
     __syncthreads();
